@@ -1,0 +1,303 @@
+// encode.hip -- k-mer hypervector encoding on gfx950.
+//
+// Semantics: HVCache::compute (src/encoder.cpp:106-132) with the k-mer HVs of
+// the HVCache constructor (src/encoder.cpp:47-74):
+//   HV(kmer)[d] = NOT( XOR_i rotR(base[c_i], i)[d] ),  c = lexmin(kmer, revcomp)
+//   count[d]    = sum over all-ACGT k-mers of (2*HV[d] - 1)
+//               = n_valid - 2 * #{k-mers whose XOR word has bit d set}.
+//
+// Design (one workgroup per sequence, persistent over sequences):
+//   * the table rot[b][i][w] (4 x k x W32 words, 61 KB at D=10240, k=12) is
+//     staged in LDS once per workgroup;
+//   * phase 1: lanes compute the canonical code of each k-mer position of a
+//     1024-position chunk from the 2-bit packed sequence (rolling-free: one
+//     64-bit window per position), written to LDS as a "digit i = c_i" code;
+//   * phase 2: lane w owns 32 dimensions (word w).  The k-mer code is wave
+//     uniform (readfirstlane -> SGPR), so the k table loads of a k-mer hit k
+//     conflict-free LDS rows; 8 XOR words are reduced by a carry-save adder
+//     tree and rippled into P bit-sliced counter planes;
+//   * phase 3: planes are transposed to per-dimension counts, written as
+//     int16 counts and/or int8 limbs, with the exact sum of squares and the
+//     max |count| reduced over the workgroup.
+#include "hvk_internal.h"
+
+namespace hvk {
+namespace {
+
+constexpr uint32_t kInvalidCode = 0xFFFFFFFFu;
+constexpr int kChunk = 1024;
+
+__device__ __forceinline__ uint32_t rev_digits(uint32_t v, int k) {
+    uint32_t y = __builtin_bitreverse32(v);
+    y = ((y >> 1) & 0x55555555u) | ((y & 0x55555555u) << 1);
+    return y >> (32 - 2 * k);
+}
+
+__device__ __forceinline__ void full_add(uint32_t a, uint32_t b, uint32_t c, uint32_t& s,
+                                         uint32_t& cy) {
+    const uint32_t t = a ^ b;
+    s = t ^ c;
+    cy = (a & b) | (t & c);
+}
+
+// planes += popcount-per-lane of x[0..7] (bit-sliced, lane = bit position)
+template <int P>
+__device__ __forceinline__ void add8(uint32_t (&planes)[P], const uint32_t (&x)[8]) {
+    uint32_t a1, a2, b1, b2, c1, c2, d1, d2, e2, e4, f2, f4, g4, g8;
+    full_add(x[0], x[1], x[2], a1, a2);
+    full_add(x[3], x[4], x[5], b1, b2);
+    full_add(x[6], x[7], a1, c1, c2);
+    d1 = b1 ^ c1;  // half adder
+    d2 = b1 & c1;
+    full_add(a2, b2, c2, e2, e4);
+    f2 = e2 ^ d2;
+    f4 = e2 & d2;
+    g4 = e4 ^ f4;
+    g8 = e4 & f4;
+    // ripple (d1 + 2 f2 + 4 g4 + 8 g8) into the planes
+    uint32_t cy = planes[0] & d1;
+    planes[0] ^= d1;
+    uint32_t s;
+    full_add(planes[1], f2, cy, s, cy);
+    planes[1] = s;
+    full_add(planes[2], g4, cy, s, cy);
+    planes[2] = s;
+    full_add(planes[3], g8, cy, s, cy);
+    planes[3] = s;
+#pragma unroll
+    for (int p = 4; p < P; ++p) {
+        const uint32_t t = planes[p] & cy;
+        planes[p] ^= cy;
+        cy = t;
+    }
+}
+
+__device__ __forceinline__ int64_t wave_sum64(int64_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        int lo = __shfl_xor((int)(uint32_t)v, o, 64);
+        int hi = __shfl_xor((int)(v >> 32), o, 64);
+        v += (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+    }
+    return v;
+}
+
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+template <int K, int P>
+__global__ void __launch_bounds__(1024)
+encode_kernel(const uint32_t* __restrict__ gtable, int k_rt, int D, int W32,
+              const uint32_t* __restrict__ packed, const uint32_t* __restrict__ invalid,
+              const int64_t* __restrict__ offsets, int n_seqs, int16_t* __restrict__ counts,
+              int8_t* __restrict__ lo, int8_t* __restrict__ hi, int64_t ld,
+              int64_t* __restrict__ sqnorm, int32_t* __restrict__ maxabs,
+              int32_t* __restrict__ nkmers) {
+    const int k = K > 0 ? K : k_rt;
+    extern __shared__ uint32_t smem[];
+    uint32_t* tab = smem;
+    uint32_t* codes = smem + 4 * k * W32;
+    __shared__ int64_t red_sq[16];
+    __shared__ int red_mx[16];
+
+    const int tid = threadIdx.x;
+    const int nthr = blockDim.x;
+    const int lane = tid & 63, wave = tid >> 6, nwaves = nthr >> 6;
+    for (int i = tid; i < 4 * k * W32; i += nthr) tab[i] = gtable[i];
+
+    const uint32_t mask2k = (k == 16) ? 0xFFFFFFFFu : ((1u << (2 * k)) - 1u);
+    const uint32_t maskk = (1u << k) - 1u;
+    const int w = tid;
+    const int kW = k * W32;
+
+    for (int s = blockIdx.x; s < n_seqs; s += gridDim.x) {
+        const int64_t off = offsets[s];
+        const int64_t len = offsets[s + 1] - off;
+        const int64_t npos = len - k + 1;
+        uint32_t planes[P];
+#pragma unroll
+        for (int p = 0; p < P; ++p) planes[p] = 0;
+        int nvalid = 0;
+
+        for (int64_t c0 = 0; c0 < npos; c0 += kChunk) {
+            const int cn = (int)((npos - c0) < kChunk ? (npos - c0) : kChunk);
+            __syncthreads();
+            for (int j = tid; j < cn; j += nthr) {
+                const int64_t p = off + c0 + j;
+                const uint64_t w2 = (uint64_t)packed[p >> 4] | ((uint64_t)packed[(p >> 4) + 1] << 32);
+                const uint32_t v = (uint32_t)(w2 >> (2 * (p & 15))) & mask2k;
+                const uint64_t iv = (uint64_t)invalid[p >> 5] | ((uint64_t)invalid[(p >> 5) + 1] << 32);
+                const uint32_t bad = (uint32_t)(iv >> (p & 31)) & maskk;
+                uint32_t code = kInvalidCode;
+                if (!bad) {
+                    // v: digit i = base i; forward code f has base 0 most significant;
+                    // revcomp code = mask - v.  Store rev(canonical): digit i = c_i.
+                    const uint32_t f = rev_digits(v, k);
+                    const uint32_t r = mask2k - v;
+                    code = (r < f) ? (mask2k - f) : v;
+                }
+                codes[j] = code;
+            }
+            __syncthreads();
+            if (w < W32) {
+                for (int j = 0; j < cn; j += 8) {
+                    uint32_t x[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        uint32_t code = (j + u < cn) ? codes[j + u] : kInvalidCode;
+                        code = __builtin_amdgcn_readfirstlane(code);
+                        uint32_t y = 0;
+                        if (code != kInvalidCode) {
+                            ++nvalid;
+                            if (K > 0) {
+#pragma unroll
+                                for (int i = 0; i < (K > 0 ? K : 1); ++i)
+                                    y ^= tab[((code >> (2 * i)) & 3u) * kW + i * W32 + w];
+                            } else {
+                                for (int i = 0; i < k; ++i)
+                                    y ^= tab[((code >> (2 * i)) & 3u) * kW + i * W32 + w];
+                            }
+                        }
+                        x[u] = y;
+                    }
+                    add8<P>(planes, x);
+                }
+            }
+        }
+
+        // ---- phase 3: per-dimension counts ----
+        int64_t sq = 0;
+        int mx = 0;
+        if (w < W32) {
+            uint32_t packed16[16];
+            uint32_t plo[8], phi[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) { plo[q] = 0; phi[q] = 0; }
+#pragma unroll
+            for (int b = 0; b < 32; ++b) {
+                uint32_t pc = 0;
+#pragma unroll
+                for (int p = 0; p < P; ++p) pc |= ((planes[p] >> b) & 1u) << p;
+                int cnt = (32 * w + b < D) ? (int)(int16_t)(nvalid - 2 * (int)pc) : 0;
+                sq += (int64_t)cnt * cnt;
+                mx = max(mx, cnt < 0 ? -cnt : cnt);
+                const uint32_t u16 = (uint32_t)(uint16_t)(int16_t)cnt;
+                if (b & 1) packed16[b >> 1] |= u16 << 16;
+                else packed16[b >> 1] = u16;
+                const int l8 = (int)(int8_t)(cnt & 0xFF);
+                const int h8 = (cnt - l8) >> 8;
+                plo[b >> 2] |= ((uint32_t)(uint8_t)l8) << (8 * (b & 3));
+                phi[b >> 2] |= ((uint32_t)(uint8_t)h8) << (8 * (b & 3));
+            }
+            const int64_t row = (int64_t)s * ld + 32 * (int64_t)w;
+            if (counts) {
+                uint4* dst = reinterpret_cast<uint4*>(counts + row);
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    dst[q] = make_uint4(packed16[4 * q], packed16[4 * q + 1], packed16[4 * q + 2],
+                                        packed16[4 * q + 3]);
+            }
+            if (lo) {
+                uint4* dst = reinterpret_cast<uint4*>(lo + row);
+                dst[0] = make_uint4(plo[0], plo[1], plo[2], plo[3]);
+                dst[1] = make_uint4(plo[4], plo[5], plo[6], plo[7]);
+            }
+            if (hi) {
+                uint4* dst = reinterpret_cast<uint4*>(hi + row);
+                dst[0] = make_uint4(phi[0], phi[1], phi[2], phi[3]);
+                dst[1] = make_uint4(phi[4], phi[5], phi[6], phi[7]);
+            }
+        }
+        // zero padding columns beyond the last full word
+        for (int64_t d = 32 * (int64_t)W32 + tid; d < ld; d += nthr) {
+            const int64_t o = (int64_t)s * ld + d;
+            if (counts) counts[o] = 0;
+            if (lo) lo[o] = 0;
+            if (hi) hi[o] = 0;
+        }
+        sq = wave_sum64(sq);
+        mx = wave_max(mx);
+        if (lane == 0) { red_sq[wave] = sq; red_mx[wave] = mx; }
+        __syncthreads();
+        if (tid == 0) {
+            int64_t t = 0;
+            int m = 0;
+            for (int q = 0; q < nwaves; ++q) { t += red_sq[q]; m = max(m, red_mx[q]); }
+            if (sqnorm) sqnorm[s] = t;
+            if (maxabs) maxabs[s] = m;
+            if (nkmers) nkmers[s] = nvalid;
+        }
+        __syncthreads();
+    }
+}
+
+template <int K, int P>
+int launch_kp(hvk_ctx* ctx, const uint32_t* packed, const uint32_t* invalid,
+              const int64_t* offsets, int n, int16_t* counts, int8_t* lo, int8_t* hi,
+              int64_t ld, int64_t* sqnorm, int32_t* maxabs, int32_t* nkmers, hipStream_t st) {
+    const int W32 = ctx->w32;
+    const int threads = (W32 + 63) / 64 * 64;
+    const size_t lds = sizeof(uint32_t) * ((size_t)4 * ctx->k * W32 + kChunk);
+    static bool attr_set = false;
+    if (!attr_set) {
+        HVK_HIP(hipFuncSetAttribute((const void*)encode_kernel<K, P>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        attr_set = true;
+    }
+    int grid = n < 4 * ctx->n_cu ? n : 4 * ctx->n_cu;
+    timing_mark(ctx, 0, false, st);
+    hipLaunchKernelGGL((encode_kernel<K, P>), dim3(grid), dim3(threads), lds, st, ctx->d_table,
+                       ctx->k, ctx->dim, W32, packed, invalid, offsets, n, counts, lo, hi, ld,
+                       sqnorm, maxabs, nkmers);
+    timing_mark(ctx, 0, true, st);
+    HVK_HIP(hipGetLastError());
+    return HVK_OK;
+}
+
+template <int K>
+int launch_k(hvk_ctx* ctx, int max_len, const uint32_t* packed, const uint32_t* invalid,
+             const int64_t* offsets, int n, int16_t* counts, int8_t* lo, int8_t* hi, int64_t ld,
+             int64_t* sqnorm, int32_t* maxabs, int32_t* nkmers, hipStream_t st) {
+    const int64_t nmax = (int64_t)max_len - ctx->k + 1;
+    if (nmax < (1 << 11))
+        return launch_kp<K, 11>(ctx, packed, invalid, offsets, n, counts, lo, hi, ld, sqnorm, maxabs, nkmers, st);
+    if (nmax < (1 << 16))
+        return launch_kp<K, 16>(ctx, packed, invalid, offsets, n, counts, lo, hi, ld, sqnorm, maxabs, nkmers, st);
+    return launch_kp<K, 24>(ctx, packed, invalid, offsets, n, counts, lo, hi, ld, sqnorm, maxabs, nkmers, st);
+}
+
+}  // namespace
+
+int launch_encode(hvk_ctx* ctx, const uint32_t* packed, const uint32_t* invalid,
+                  const int64_t* offsets, int n, int max_len, int16_t* counts, int8_t* lo,
+                  int8_t* hi, int64_t ld, int64_t* sqnorm, int32_t* maxabs, int32_t* nkmers,
+                  hipStream_t st) {
+    switch (ctx->k) {
+        case 5: return launch_k<5>(ctx, max_len, packed, invalid, offsets, n, counts, lo, hi, ld, sqnorm, maxabs, nkmers, st);
+        case 10: return launch_k<10>(ctx, max_len, packed, invalid, offsets, n, counts, lo, hi, ld, sqnorm, maxabs, nkmers, st);
+        case 12: return launch_k<12>(ctx, max_len, packed, invalid, offsets, n, counts, lo, hi, ld, sqnorm, maxabs, nkmers, st);
+        default: return launch_k<0>(ctx, max_len, packed, invalid, offsets, n, counts, lo, hi, ld, sqnorm, maxabs, nkmers, st);
+    }
+}
+
+}  // namespace hvk
+
+extern "C" int hvk_encode(hvk_ctx* ctx, const uint32_t* d_packed, const uint32_t* d_invalid,
+                          const int64_t* d_offsets, int n_seqs, int max_len, int16_t* d_counts,
+                          int8_t* d_lo, int8_t* d_hi, int64_t ld, int64_t* d_sqnorm,
+                          int32_t* d_maxabs, int32_t* d_nkmers, hvk_stream stream) {
+    if (!ctx || !d_packed || !d_invalid || !d_offsets || n_seqs < 0 || max_len < 0)
+        return HVK_ERR_ARG;
+    if (n_seqs == 0) return HVK_OK;
+    if (ld == 0) ld = ctx->ld;
+    if (ld < 32 * (int64_t)ctx->w32 || (ld % 16) != 0) return HVK_ERR_ARG;
+    if ((int64_t)max_len - ctx->k + 1 >= (1 << 24)) return HVK_ERR_RANGE;
+    // LDS budget: rotated table + one chunk of codes
+    if ((size_t)4 * ctx->k * ctx->w32 * 4 + 4096 > 150 * 1024) return HVK_ERR_ARG;
+    return hvk::launch_encode(ctx, d_packed, d_invalid, d_offsets, n_seqs, max_len, d_counts,
+                              d_lo, d_hi, ld, d_sqnorm, d_maxabs, d_nkmers,
+                              (hipStream_t)stream);
+}

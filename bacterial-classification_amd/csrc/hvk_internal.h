@@ -1,0 +1,76 @@
+// hvk_internal.h -- shared definitions of the gfx950 HDC library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <mutex>
+#include <vector>
+
+#include "../../include/hdc_amd.h"
+
+namespace hvk {
+
+constexpr int kMaxDim = 32768;
+constexpr int kMaxK = 16;
+constexpr int kMaxTopK = 32;
+
+// Row stride of the limb / count outputs: D rounded up to 128 elements so a
+// K-slab of any search kernel never straddles a row end.
+inline int64_t row_ld_for(int dim) { return (int64_t(dim) + 127) / 128 * 128; }
+
+void set_last_error(const char* where, hipError_t e);
+
+#define HVK_HIP(call)                                   \
+    do {                                                \
+        hipError_t e_ = (call);                         \
+        if (e_ != hipSuccess) {                         \
+            ::hvk::set_last_error(#call, e_);           \
+            return HVK_ERR_HIP;                         \
+        }                                               \
+    } while (0)
+
+// A device buffer that only grows; reused across calls on one context.
+struct Workspace {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t need);
+    ~Workspace();
+};
+
+}  // namespace hvk
+
+struct hvk_ctx {
+    int dim = 0;
+    int k = 0;
+    int w64 = 0;   // words of 64 bits per binary HV
+    int w32 = 0;   // words of 32 bits
+    int64_t ld = 0;
+    int device = 0;
+    int n_cu = 0;
+    std::vector<uint64_t> base[4];      // A C G T (code order), W words each
+    uint32_t* d_table = nullptr;        // [4][k][w32] rotated base words
+    hvk::Workspace ws_search;           // partial top lists
+    hvk::Workspace ws_norms;            // inverse norms
+    hvk::Workspace ws_flags;            // rescan queue
+    std::mutex mu;
+    // kernel timing (hvk_ctx_set_timing)
+    bool timing = false;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool ev_used[2] = {false, false};  // encode, search
+};
+
+// Kernel launchers (defined in the .hip files).
+namespace hvk {
+// which = 0: encode, 1: search.  Records the start / stop event when timing.
+inline void timing_mark(hvk_ctx* c, int which, bool stop, hipStream_t s) {
+    if (!c->timing) return;
+    (void)hipEventRecord(c->ev[2 * which + (stop ? 1 : 0)], s);
+    if (stop) c->ev_used[which] = true;
+}
+int launch_encode(hvk_ctx* ctx, const uint32_t* packed, const uint32_t* invalid,
+                  const int64_t* offsets, int n, int max_len, int16_t* counts,
+                  int8_t* lo, int8_t* hi, int64_t ld, int64_t* sqnorm, int32_t* maxabs,
+                  int32_t* nkmers, hipStream_t s);
+}

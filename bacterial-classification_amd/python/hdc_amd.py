@@ -1,0 +1,299 @@
+"""ctypes binding of libhdc_amd.so (the gfx950 HDC read-classification path).
+
+Thin plumbing for tests and the benchmark: device buffers are torch tensors,
+kernels run through the C ABI declared in include/hdc_amd.h.  There is no
+fallback: if the library is missing or has no GPU, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "lib", "libhdc_amd.so")
+
+HVK_OK = 0
+HVK_ERR_ARG = 1
+HVK_ERR_HIP = 2
+HVK_ERR_RANGE = 3
+HVK_ERR_NOMEM = 4
+HVK_ERR_NODEVICE = 5
+
+# entry points declared in include/hdc_amd.h
+EXPORTS = [
+    "hvk_status_string", "hvk_last_error", "hvk_ctx_create", "hvk_ctx_destroy",
+    "hvk_ctx_info", "hvk_base_hv", "hvk_pack_words", "hvk_pack", "hvk_encode",
+    "hvk_search_plain", "hvk_search_mfma", "hvk_gen_size", "hvk_gen_fill",
+    "hvk_ctx_set_timing", "hvk_ctx_last_timing",
+]
+
+
+class HvkError(RuntimeError):
+    def __init__(self, status: int, what: str, lib=None):
+        msg = f"{what}: status {status}"
+        if lib is not None:
+            msg += f" ({lib.hvk_status_string(status).decode()}; {lib.hvk_last_error().decode()})"
+        super().__init__(msg)
+        self.status = status
+
+
+class GenSpec(ctypes.Structure):
+    _fields_ = [
+        ("seed", ctypes.c_uint64),
+        ("n_ancestors", ctypes.c_int32),
+        ("desc_per_ancestor", ctypes.c_int32),
+        ("anc_len_min", ctypes.c_int32),
+        ("anc_len_max", ctypes.c_int32),
+        ("div_min", ctypes.c_double),
+        ("div_max", ctypes.c_double),
+        ("indel_rate", ctypes.c_double),
+        ("n_reads", ctypes.c_int32),
+        ("read_len_min", ctypes.c_int32),
+        ("read_len_max", ctypes.c_int32),
+        ("read_len_log_uniform", ctypes.c_int32),
+        ("err_min", ctypes.c_double),
+        ("err_max", ctypes.c_double),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load libhdc_amd.so (raises when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(f"{LIB_PATH} missing: run __graft_entry__.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i32, i64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64
+    L.hvk_status_string.restype = ctypes.c_char_p
+    L.hvk_status_string.argtypes = [i32]
+    L.hvk_last_error.restype = ctypes.c_char_p
+    L.hvk_last_error.argtypes = []
+    L.hvk_ctx_create.argtypes = [i32, i32, ctypes.POINTER(vp)]
+    L.hvk_ctx_destroy.argtypes = [vp]
+    L.hvk_ctx_info.argtypes = [vp] + [ctypes.POINTER(ctypes.c_int)] * 4
+    L.hvk_base_hv.argtypes = [vp, ctypes.c_char, vp]
+    L.hvk_pack_words.restype = i64
+    L.hvk_pack_words.argtypes = [i64]
+    L.hvk_pack.argtypes = [vp, vp, i32, vp, vp]
+    L.hvk_encode.argtypes = [vp, vp, vp, vp, i32, i32, vp, vp, vp, i64, vp, vp, vp, vp]
+    L.hvk_search_plain.argtypes = [vp, vp, vp, i32, i32, vp, vp, i32, i32, i64, i32,
+                                   vp, vp, vp, vp, vp]
+    L.hvk_search_mfma.argtypes = [vp, vp, vp, vp, i32, vp, vp, vp, i32, i64, i32,
+                                  vp, vp, vp, vp, vp]
+    L.hvk_ctx_set_timing.argtypes = [vp, i32]
+    L.hvk_ctx_last_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
+    L.hvk_gen_size.argtypes = [ctypes.POINTER(GenSpec), ctypes.POINTER(i64), ctypes.POINTER(i64)]
+    L.hvk_gen_fill.argtypes = [ctypes.POINTER(GenSpec), vp, vp, vp, vp, vp]
+    _lib = L
+    return L
+
+
+def check(status: int, what: str):
+    if status != HVK_OK:
+        raise HvkError(status, what, lib())
+
+
+# ----------------------------------------------------------------------------
+# host helpers (no GPU)
+# ----------------------------------------------------------------------------
+def concat(seqs) -> tuple[bytes, np.ndarray]:
+    """list[str|bytes] -> (concatenated bytes, int64 offsets[n+1])."""
+    bs = [s.encode() if isinstance(s, str) else bytes(s) for s in seqs]
+    off = np.zeros(len(bs) + 1, np.int64)
+    if bs:
+        off[1:] = np.cumsum([len(b) for b in bs])
+    return b"".join(bs), off
+
+
+def pack(buf: bytes, offsets: np.ndarray):
+    """2-bit codes + invalid-base mask (numpy uint32 arrays)."""
+    L = lib()
+    total = int(offsets[-1])
+    words = int(L.hvk_pack_words(total))
+    packed = np.zeros(words, np.uint32)
+    invalid = np.zeros(words, np.uint32)
+    cbuf = ctypes.create_string_buffer(buf, max(len(buf), 1))
+    off = np.ascontiguousarray(offsets, np.int64)
+    check(L.hvk_pack(cbuf, off.ctypes.data, len(off) - 1, packed.ctypes.data,
+                     invalid.ctypes.data), "hvk_pack")
+    return packed, invalid
+
+
+@dataclass
+class Workload:
+    refs: bytes
+    ref_offsets: np.ndarray
+    reads: bytes
+    read_offsets: np.ndarray
+    truth: np.ndarray
+
+
+def make_spec(**kw) -> GenSpec:
+    d = dict(seed=0, n_ancestors=20, desc_per_ancestor=50, anc_len_min=1500, anc_len_max=1500,
+             div_min=0.03, div_max=0.03, indel_rate=0.01, n_reads=500, read_len_min=250,
+             read_len_max=250, read_len_log_uniform=0, err_min=0.01, err_max=0.01)
+    d.update(kw)
+    return GenSpec(**d)
+
+
+def generate(spec: GenSpec) -> Workload:
+    L = lib()
+    a, b = ctypes.c_int64(), ctypes.c_int64()
+    check(L.hvk_gen_size(ctypes.byref(spec), ctypes.byref(a), ctypes.byref(b)), "hvk_gen_size")
+    n_refs = spec.n_ancestors * spec.desc_per_ancestor
+    refs = ctypes.create_string_buffer(max(a.value, 1))
+    reads = ctypes.create_string_buffer(max(b.value, 1))
+    ro = np.zeros(n_refs + 1, np.int64)
+    qo = np.zeros(spec.n_reads + 1, np.int64)
+    truth = np.zeros(spec.n_reads, np.int32)
+    check(L.hvk_gen_fill(ctypes.byref(spec), refs, ro.ctypes.data, reads, qo.ctypes.data,
+                         truth.ctypes.data), "hvk_gen_fill")
+    return Workload(refs.raw[: a.value], ro, reads.raw[: b.value], qo, truth)
+
+
+# ----------------------------------------------------------------------------
+# device side
+# ----------------------------------------------------------------------------
+def _torch():
+    import torch
+    return torch
+
+
+def _ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _stream():
+    torch = _torch()
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class Context:
+    """D, k and the device tables (hvk_ctx)."""
+
+    def __init__(self, dim: int, kmer: int):
+        L = lib()
+        h = ctypes.c_void_p()
+        check(L.hvk_ctx_create(dim, kmer, ctypes.byref(h)), "hvk_ctx_create")
+        self.h = h
+        d, k, w, ld = (ctypes.c_int() for _ in range(4))
+        check(L.hvk_ctx_info(h, ctypes.byref(d), ctypes.byref(k), ctypes.byref(w), ctypes.byref(ld)),
+              "hvk_ctx_info")
+        self.dim, self.kmer, self.words, self.ld = d.value, k.value, w.value, ld.value
+
+    def close(self):
+        if self.h:
+            lib().hvk_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_timing(self, enable: bool):
+        check(lib().hvk_ctx_set_timing(self.h, int(enable)), "hvk_ctx_set_timing")
+
+    def last_timing(self):
+        """(encode_ms, search_ms) of the main kernels of the last calls."""
+        e, s = ctypes.c_float(), ctypes.c_float()
+        check(lib().hvk_ctx_last_timing(self.h, ctypes.byref(e), ctypes.byref(s)),
+              "hvk_ctx_last_timing")
+        return e.value, s.value
+
+    def base_hv(self, symbol: str) -> np.ndarray:
+        out = np.zeros(self.words, np.uint64)
+        check(lib().hvk_base_hv(self.h, symbol.encode(), out.ctypes.data), "hvk_base_hv")
+        return out
+
+
+@dataclass
+class Encoded:
+    n: int
+    counts: object   # torch int16 [n][ld] or None
+    lo: object       # torch int8 [n][ld] or None
+    hi: object
+    sqnorm: object   # torch int64 [n]
+    maxabs: object   # torch int32 [n]
+    nkmers: object   # torch int32 [n]
+
+
+class DeviceSeqs:
+    """Packed sequences resident on the device."""
+
+    def __init__(self, buf: bytes, offsets: np.ndarray):
+        torch = _torch()
+        packed, invalid = pack(buf, offsets)
+        self.n = len(offsets) - 1
+        lens = np.diff(offsets)
+        self.max_len = int(lens.max()) if self.n else 0
+        self.packed = torch.from_numpy(packed.view(np.int32)).cuda()
+        self.invalid = torch.from_numpy(invalid.view(np.int32)).cuda()
+        self.offsets = torch.from_numpy(np.ascontiguousarray(offsets, np.int64)).cuda()
+
+
+def encode(ctx: Context, seqs: DeviceSeqs, counts=True, limbs=True, hi=True) -> Encoded:
+    torch = _torch()
+    n, ld = seqs.n, ctx.ld
+    dev = "cuda"
+    c = torch.empty((n, ld), dtype=torch.int16, device=dev) if counts else None
+    lo = torch.empty((n, ld), dtype=torch.int8, device=dev) if limbs else None
+    h = torch.empty((n, ld), dtype=torch.int8, device=dev) if (limbs and hi) else None
+    sq = torch.empty(n, dtype=torch.int64, device=dev)
+    mx = torch.empty(n, dtype=torch.int32, device=dev)
+    nk = torch.empty(n, dtype=torch.int32, device=dev)
+    check(lib().hvk_encode(ctx.h, _ptr(seqs.packed), _ptr(seqs.invalid), _ptr(seqs.offsets), n,
+                           seqs.max_len, _ptr(c), _ptr(lo), _ptr(h), ld, _ptr(sq), _ptr(mx),
+                           _ptr(nk), _stream()), "hvk_encode")
+    return Encoded(n, c, lo, h, sq, mx, nk)
+
+
+@dataclass
+class TopK:
+    idx: object     # torch int32 [n][k]
+    score: object   # torch float32 [n][k]
+    cos64: object   # torch float64 [n][k]
+    n_rescored: object  # torch int32 [1]
+
+
+def _topk_bufs(n, k):
+    torch = _torch()
+    return TopK(torch.empty((n, k), dtype=torch.int32, device="cuda"),
+                torch.empty((n, k), dtype=torch.float32, device="cuda"),
+                torch.empty((n, k), dtype=torch.float64, device="cuda"),
+                torch.zeros(1, dtype=torch.int32, device="cuda"))
+
+
+def search_plain(ctx: Context, q: Encoded, r: Encoded, k: int, q_maxabs=None, r_maxabs=None,
+                 out: TopK | None = None) -> TopK:
+    qm = int(q.maxabs.max().item()) if q_maxabs is None else q_maxabs
+    rm = int(r.maxabs.max().item()) if r_maxabs is None else r_maxabs
+    o = out or _topk_bufs(q.n, k)
+    check(lib().hvk_search_plain(ctx.h, _ptr(q.counts), _ptr(q.sqnorm), q.n, qm, _ptr(r.counts),
+                                 _ptr(r.sqnorm), r.n, rm, ctx.ld, k, _ptr(o.idx), _ptr(o.score),
+                                 _ptr(o.cos64), _ptr(o.n_rescored), _stream()), "hvk_search_plain")
+    return o
+
+
+def search_mfma(ctx: Context, q: Encoded, r: Encoded, k: int, q_two_limbs=None, r_two_limbs=None,
+                out: TopK | None = None) -> TopK:
+    """q_two_limbs/r_two_limbs: None = decide from max |count| (> 127)."""
+    if q_two_limbs is None:
+        q_two_limbs = int(q.maxabs.max().item()) > 127
+    if r_two_limbs is None:
+        r_two_limbs = int(r.maxabs.max().item()) > 127
+    o = out or _topk_bufs(q.n, k)
+    check(lib().hvk_search_mfma(ctx.h, _ptr(q.lo), _ptr(q.hi) if q_two_limbs else None,
+                                _ptr(q.sqnorm), q.n, _ptr(r.lo),
+                                _ptr(r.hi) if r_two_limbs else None, _ptr(r.sqnorm), r.n, ctx.ld,
+                                k, _ptr(o.idx), _ptr(o.score), _ptr(o.cos64), _ptr(o.n_rescored),
+                                _stream()), "hvk_search_mfma")
+    return o

@@ -1,0 +1,217 @@
+#!/usr/bin/env python3
+"""Benchmark: reads/s classified (D=10240, k=12, 100k references, top-5).
+
+One step = encode a batch of reads on the GPU (HVCache::compute semantics)
+and search it against the 100k reference prototypes with the MFMA form,
+returning the top-5 (index, cosine) per read.  Inputs are resident in HBM
+before the timed region; the reference database is encoded once, outside it.
+
+Also measured in the same run: the plain (VALU) form on a subset of reads,
+the main-kernel durations (HIP events on the launch stream) for the roofline,
+and the CPU baseline (the C restatement of the reference's bit-packed path,
+timed on this host) on a bounded sample, whose top-5 must equal the GPU's.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "bacterial-classification_amd", "python"))
+
+DIM, KMER, TOPK = 10240, 12, 5
+MFMA_INT8_PEAK_TOPS = 5000.0  # MI355X dense int8 MFMA, TOPS
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--reads", type=int, default=1_000_000)
+    ap.add_argument("--ancestors", type=int, default=2000)
+    ap.add_argument("--plain-reads", type=int, default=16384)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    return ap.parse_args()
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(see DESIGN.md for how the search shards).", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import hdc_amd as H
+
+    torch.cuda.set_device(0)
+    t0 = time.time()
+    spec = H.make_spec(seed=0, n_ancestors=a.ancestors, desc_per_ancestor=50, anc_len_min=1300,
+                       anc_len_max=1600, div_min=0.02, div_max=0.08, indel_rate=0.01,
+                       n_reads=a.reads, read_len_min=150, read_len_max=300, err_min=0.005,
+                       err_max=0.02)
+    w = H.generate(spec)
+    n_refs = len(w.ref_offsets) - 1
+    print(f"[bench] generated {n_refs} refs, {a.reads} reads in {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+
+    ctx = H.Context(DIM, KMER)
+    ctx.set_timing(True)
+    ref_seqs = H.DeviceSeqs(w.refs, w.ref_offsets)
+    torch.cuda.synchronize()
+    t1 = time.time()
+    db = H.encode(ctx, ref_seqs, counts=True, limbs=True, hi=True)
+    torch.cuda.synchronize()
+    db_encode_s = time.time() - t1
+    db_two_limbs = int(db.maxabs.max().item()) > 127
+    read_seqs = H.DeviceSeqs(w.reads, w.read_offsets)
+    torch.cuda.synchronize()
+    print(f"[bench] DB encoded in {db_encode_s:.2f}s (max|count| {int(db.maxabs.max())})",
+          file=sys.stderr, flush=True)
+
+    nq, ld = a.reads, ctx.ld
+    q_lo = torch.empty((nq, ld), dtype=torch.int8, device="cuda")
+    q_hi = torch.empty((nq, ld), dtype=torch.int8, device="cuda")
+    q_sq = torch.empty(nq, dtype=torch.int64, device="cuda")
+    q_mx = torch.empty(nq, dtype=torch.int32, device="cuda")
+    q_nk = torch.empty(nq, dtype=torch.int32, device="cuda")
+    out = H._topk_bufs(nq, TOPK)
+    L = H.lib()
+
+    def step():
+        H.check(L.hvk_encode(ctx.h, H._ptr(read_seqs.packed), H._ptr(read_seqs.invalid),
+                             H._ptr(read_seqs.offsets), nq, read_seqs.max_len, None,
+                             H._ptr(q_lo), H._ptr(q_hi), ld, H._ptr(q_sq), H._ptr(q_mx),
+                             H._ptr(q_nk), H._stream()), "hvk_encode")
+        enc = H.Encoded(nq, None, q_lo, q_hi, q_sq, q_mx, q_nk)
+        H.search_mfma(ctx, enc, db, TOPK, q_two_limbs=q_two, r_two_limbs=db_two_limbs, out=out)
+
+    # the limb count of the read side is fixed by the data: decide once
+    H.check(L.hvk_encode(ctx.h, H._ptr(read_seqs.packed), H._ptr(read_seqs.invalid),
+                         H._ptr(read_seqs.offsets), nq, read_seqs.max_len, None, H._ptr(q_lo),
+                         H._ptr(q_hi), ld, H._ptr(q_sq), H._ptr(q_mx), H._ptr(q_nk),
+                         H._stream()), "hvk_encode")
+    torch.cuda.synchronize()
+    q_two = int(q_mx.max().item()) > 127
+    for i in range(a.warmup):
+        step()
+        torch.cuda.synchronize()
+        print(f"[bench] warmup {i} done", file=sys.stderr, flush=True)
+
+    enc_ms, srch_ms, step_s = [], [], []
+    torch.cuda.synchronize()
+    t_start = time.time()
+    for i in range(a.steps):
+        ts = time.time()
+        step()
+        torch.cuda.synchronize()
+        step_s.append(time.time() - ts)
+        e, s = ctx.last_timing()
+        enc_ms.append(e)
+        srch_ms.append(s)
+        print(f"[bench] step {i}: {step_s[-1]:.3f}s encode {e:.1f}ms search {s:.1f}ms",
+              file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    total = time.time() - t_start
+    ms_per_step = 1000.0 * total / a.steps
+    value = nq / (total / a.steps)
+    n_res = int(out.n_rescored.item())
+
+    # roofline of the dominant kernel (the MFMA search)
+    s_kernel = float(np.mean(srch_ms)) / 1000.0
+    algo_ops = 2.0 * DIM * n_refs * nq  # logical MACs x2, limb passes not counted
+    achieved_tops = algo_ops / s_kernel / 1e12
+    e_kernel = float(np.mean(enc_ms)) / 1000.0
+    lens = np.diff(w.read_offsets)
+    enc_bytes = float(np.sum((lens + 3) // 4)) + nq * (2.0 * ld) + nq * 8  # in + limbs + norm
+    enc_gbs = enc_bytes / e_kernel / 1e9
+
+    # plain form on a subset
+    plain = None
+    if not a.no_plain:
+        npl = min(a.plain_reads, nq)
+        sub = H.DeviceSeqs(w.reads[: w.read_offsets[npl]], w.read_offsets[: npl + 1])
+        qp = H.encode(ctx, sub, counts=True, limbs=False)
+        torch.cuda.synchronize()
+        H.search_plain(ctx, qp, db, TOPK)  # warm
+        torch.cuda.synchronize()
+        tp = time.time()
+        resp = H.search_plain(ctx, qp, db, TOPK)
+        torch.cuda.synchronize()
+        plain_s = time.time() - tp
+        _, plain_kernel_ms = ctx.last_timing()
+        same = bool(torch.equal(resp.idx, out.idx[:npl]))
+        plain = {"reads": npl, "reads_per_s": npl / plain_s,
+                 "kernel_ms": plain_kernel_ms, "top5_equal_mfma": same,
+                 "valu_int_tops": 2.0 * DIM * n_refs * npl / (plain_kernel_ms / 1000.0) / 1e12}
+        print(f"[bench] plain: {npl / plain_s:.0f} reads/s (top-5 equal: {same})",
+              file=sys.stderr, flush=True)
+
+    # CPU baseline: the oracle's restatement of the reference's bit-packed path
+    cpu = None
+    if not a.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py as orc
+        ncores = min(16, os.cpu_count() or 1)
+        db_host = db.counts.cpu().numpy()
+        if ld != DIM:
+            db_host = np.ascontiguousarray(db_host[:, :DIM])
+        dbn = db.sqnorm.cpu().numpy()
+        O = orc.oracle()
+
+        def cpu_run(n):
+            seqs = w.reads[: w.read_offsets[n]]
+            offs = np.ascontiguousarray(w.read_offsets[: n + 1])
+            oi = np.zeros((n, TOPK), np.int32)
+            oc = np.zeros((n, TOPK), np.float64)
+            sec = O.orc_classify_reads(DIM, KMER, seqs, offs.ctypes.data, n, db_host.ctypes.data,
+                                       dbn.ctypes.data, n_refs, TOPK, oi.ctypes.data,
+                                       oc.ctypes.data, ncores)
+            return sec, oi, oc
+
+        sec1, _, _ = cpu_run(ncores)  # one read per thread: calibrates the sample size
+        per_read = sec1 / ncores
+        n_cpu = int(max(ncores, min(nq, (a.cpu_seconds / per_read) // ncores * ncores)))
+        sec, oi, oc = cpu_run(n_cpu)
+        gpu_idx = out.idx[:n_cpu].cpu().numpy()
+        cpu = {"value": n_cpu / sec, "unit": "reads/s", "cores": ncores, "kind": "port",
+               "sample": f"first {n_cpu} reads x all {n_refs} refs, top-5 "
+                         f"({sec:.1f}s wall, {ncores} threads)",
+               "top5_equal_gpu": bool(np.array_equal(oi, gpu_idx))}
+        print(f"[bench] cpu: {n_cpu / sec:.1f} reads/s on {ncores} threads, top-5 equal "
+              f"{cpu['top5_equal_gpu']}", file=sys.stderr, flush=True)
+
+    line = {
+        "metric": "reads/s classified (D=10240, 100k refs, top-5): MFMA form vs plain; % roofline",
+        "value": value, "unit": "reads/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int8",
+        "data": "synthetic (deterministic generator, seed 0)",
+        "config": {"workload": f"config 1: D={DIM}, k={KMER}, {n_refs} refs "
+                               f"(2,000 ancestors x 50, 1,300-1,600 nt, 2-8% div, 1% indel), "
+                               f"{nq} reads 150-300 nt, 0.5-2% error, top-{TOPK}",
+                   "reads": nq, "refs": n_refs, "dim": DIM, "k": KMER, "topk": TOPK,
+                   "read_limbs": 2 if q_two else 1, "ref_limbs": 2 if db_two_limbs else 1},
+        "roofline": {"bound": "mfma", "achieved": achieved_tops, "peak": MFMA_INT8_PEAK_TOPS,
+                     "unit": "TFLOP/s", "frac": achieved_tops / MFMA_INT8_PEAK_TOPS,
+                     "traffic": None, "kernel": "mfma_kernel",
+                     "kernel_ms": float(np.mean(srch_ms))},
+        "encode": {"kernel_ms": float(np.mean(enc_ms)), "achieved_gbs": enc_gbs,
+                   "frac_hbm": enc_gbs / HBM_PEAK_GBS},
+        "plain": plain,
+        "mfma_vs_plain": (value / plain["reads_per_s"]) if plain else None,
+        "n_rescored": n_res,
+        "db_encode_s": db_encode_s,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

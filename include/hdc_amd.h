@@ -1,0 +1,173 @@
+/*
+ * hdc_amd.h -- C ABI of the MI355X (gfx950) read-classification hot path:
+ * k-mer hyperdimensional encoding of DNA sequences and exact cosine top-k
+ * search against a reference prototype database.
+ *
+ * Every entry point takes plain pointers and sizes.  Pointers named d_* are
+ * device memory owned by the caller; outputs are caller-allocated, as with
+ * the reference's launchers.  A stream argument is a hipStream_t (NULL = the
+ * default stream); work is enqueued on it and the call returns without
+ * synchronising unless stated.  No C++ exception crosses this boundary: every
+ * call returns an hvk status code (HVK_OK = 0).
+ *
+ * Reference interfaces replaced (paths in the reference repository):
+ *   src/config.h:5-17              compile-time D, k        -> hvk_ctx_create
+ *   src/HV.cpp:20-51               base HVs, get_base_hv    -> hvk_base_hv
+ *   src/encoder.cpp:47-74          4^k k-mer table (ctor)   -> device base table
+ *   src/encode.cu:169-185          pack_sequence            -> hvk_pack
+ *   src/encode.cu:136 (launch_hdc_encode), src/encoder.cpp:106-132
+ *                                  (HVCache::compute)        -> hvk_encode
+ *   src/cosine.cu:441 (launch_cosine_topk), :172 (launch_cosine_argmax)
+ *                                  -> hvk_search_plain / hvk_search_mfma
+ *   src/cosine.cu:35-50            compute_inv_norms_kernel -> norms from hvk_encode
+ */
+#ifndef HDC_AMD_H
+#define HDC_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define HVK_OK               0
+#define HVK_ERR_ARG          1  /* bad argument (null, size, unsupported D/k) */
+#define HVK_ERR_HIP          2  /* a HIP runtime call failed */
+#define HVK_ERR_RANGE        3  /* values outside the exact-arithmetic range */
+#define HVK_ERR_NOMEM        4  /* device allocation failed */
+#define HVK_ERR_NODEVICE     5  /* no usable gfx950 device */
+
+typedef struct hvk_ctx hvk_ctx;
+typedef void* hvk_stream; /* hipStream_t */
+
+/* Human-readable text for a status code (static storage). */
+const char* hvk_status_string(int status);
+/* Text of the last HIP error seen by this thread's calls (static storage). */
+const char* hvk_last_error(void);
+
+/* ---- context ----------------------------------------------------------
+ * Binds dimension D (bits per hypervector) and k-mer size k on the current
+ * device: draws the base HVs exactly as the reference (std::mt19937_64 with
+ * seed A:0 C:1 T:2 G:3, W = ceil(D/64) words, tail masked) and uploads the
+ * table of their ring rotations by 0..k-1 bits used by the encoder.
+ * Supported: 64 <= D <= 32768, 1 <= k <= 16.
+ * Replaces src/config.h:5-17 and HVCache::HVCache (src/encoder.cpp:47-74). */
+int hvk_ctx_create(int dim, int kmer, hvk_ctx** out);
+int hvk_ctx_destroy(hvk_ctx* ctx);
+/* D, k, words per binary HV, and the row stride (elements) hvk_encode uses
+ * for limb/count outputs when the caller passes ld = 0. */
+int hvk_ctx_info(const hvk_ctx* ctx, int* dim, int* kmer, int* words64, int* row_ld);
+/* get_base_hv (src/HV.cpp:43-51): W words for 'A','C','G','T'; any other
+ * character gives the all-ones HV (src/HV.cpp:9-18). */
+int hvk_base_hv(const hvk_ctx* ctx, char symbol, uint64_t* host_out);
+
+/* Kernel timing: when enabled, hvk_encode and the search calls record HIP
+ * events on their stream around their main kernel; hvk_ctx_last_timing
+ * waits for those events and returns the durations in milliseconds (a value
+ * is -1 when that kernel has not run since timing was enabled). */
+int hvk_ctx_set_timing(hvk_ctx* ctx, int enable);
+int hvk_ctx_last_timing(hvk_ctx* ctx, float* encode_ms, float* search_ms);
+
+/* ---- packing (host) -----------------------------------------------------
+ * Sequences are concatenated: sequence i is seqs[offsets[i] .. offsets[i+1]).
+ * packed: 2-bit codes A0 C1 G2 T3, base p at bits 2*(p%16) of word p/16.
+ * invalid: bit p%32 of word p/32 set when base p is not one of "ACGT"
+ * (lower case and IUPAC codes included), so k-mers over it are skipped as
+ * HVCache::compute skips them (src/encoder.cpp:115-117).
+ * hvk_pack_words gives the uint32 count for each array (includes padding).
+ * Replaces pack_sequence (src/encode.cu:169-185). */
+int64_t hvk_pack_words(int64_t total_bases);
+int hvk_pack(const char* seqs, const int64_t* offsets, int n_seqs,
+             uint32_t* packed, uint32_t* invalid);
+
+/* ---- encode -----------------------------------------------------------
+ * One bundle per sequence with HVCache::compute semantics
+ * (src/encoder.cpp:106-132): count[d] = sum over the sequence's all-ACGT
+ * k-mers of (2*bit_d(HV(kmer)) - 1), HV(kmer) bound from the canonical
+ * k-mer (src/encoder.cpp:47-74).  Bit-exact with the reference.
+ *
+ * d_packed/d_invalid: hvk_pack output on the device.  d_offsets: n_seqs+1
+ * int64 base offsets on the device.  max_len: longest sequence (bases).
+ * Outputs (each nullable; row stride ld elements, ld = 0 -> ctx row_ld,
+ * padding columns [D, ld) are written as zero):
+ *   d_counts  int16 [n][ld]   counts (int16 wrap as the reference's HV_16)
+ *   d_lo,d_hi int8  [n][ld]   limbs: count = 256*hi + lo, lo in [-128,127]
+ *   d_sqnorm  int64 [n]       exact sum of count^2
+ *   d_maxabs  int32 [n]       max |count|
+ *   d_nkmers  int32 [n]       number of k-mers bundled
+ * Replaces launch_hdc_encode (src/encode.cu:136) / encode (src/GPU.cuh:6). */
+int hvk_encode(hvk_ctx* ctx, const uint32_t* d_packed, const uint32_t* d_invalid,
+               const int64_t* d_offsets, int n_seqs, int max_len,
+               int16_t* d_counts, int8_t* d_lo, int8_t* d_hi, int64_t ld,
+               int64_t* d_sqnorm, int32_t* d_maxabs, int32_t* d_nkmers,
+               hvk_stream stream);
+
+/* ---- search -----------------------------------------------------------
+ * Top-k cosine search of n_q queries against n_r references.  The order is
+ * (c desc, reference index asc) with c = dot / (sqrt(|q|^2) sqrt(|r|^2))
+ * evaluated in FP64 from the exact integer dot, and c = 0 when a norm is
+ * zero (the reference's cosine returns 2c-1 = -1 there, src/HV.cpp:236-237).
+ * Candidates are selected in FP32; every candidate whose FP32 margin is not
+ * conclusive is re-scored exactly, so the indices equal the FP64 brute-force
+ * oracle's.  Outputs per query (row-major [n_q][k]):
+ *   d_idx int32 reference index (-1 past n_r), d_score float c (the scale of
+ *   the reference's cosine_search, src/cosine.cu:270), d_cos64 (nullable)
+ *   double c.
+ * d_n_rescored (nullable, int32 on the device): count of queries that took
+ * the exact re-scan.  k <= 32.
+ *
+ * Plain form (VALU int16 dot): rows of int16 counts with stride ld (0 ->
+ * ctx row_ld).  Requires max|q| * max|r| * 64 < 2^31; the caller passes the
+ * max |count| of each side (from hvk_encode's d_maxabs) and gets
+ * HVK_ERR_RANGE otherwise.
+ * Replaces launch_cosine_topk (src/cosine.cu:441). */
+int hvk_search_plain(hvk_ctx* ctx,
+                     const int16_t* d_q, const int64_t* d_q_sqnorm, int n_q, int q_maxabs,
+                     const int16_t* d_r, const int64_t* d_r_sqnorm, int n_r, int r_maxabs,
+                     int64_t ld, int k,
+                     int32_t* d_idx, float* d_score, double* d_cos64,
+                     int32_t* d_n_rescored, hvk_stream stream);
+
+/* MFMA form: int8 limb GEMM on the matrix cores with int32 accumulation,
+ * limbs recombined exactly in int64 (value = 256*hi + lo).  d_q_hi / d_r_hi
+ * may be NULL when every |value| on that side is <= 127 (one limb); bipolar
+ * (+1/-1) HVs pass as one limb.  Limb rows have stride ld (0 -> ctx row_ld).
+ * Replaces launch_cosine_topk (src/cosine.cu:441). */
+int hvk_search_mfma(hvk_ctx* ctx,
+                    const int8_t* d_q_lo, const int8_t* d_q_hi, const int64_t* d_q_sqnorm, int n_q,
+                    const int8_t* d_r_lo, const int8_t* d_r_hi, const int64_t* d_r_sqnorm, int n_r,
+                    int64_t ld, int k,
+                    int32_t* d_idx, float* d_score, double* d_cos64,
+                    int32_t* d_n_rescored, hvk_stream stream);
+
+/* ---- synthetic workloads (host, deterministic) -------------------------
+ * Ancestors of uniform ACGT, descendants by substitution + indels, and reads
+ * sampled from descendants with substitution errors; every draw comes from
+ * std::mt19937_64 streams derived from `seed`, so a spec always yields the
+ * same data.  Two calls: *_size then *_fill. */
+typedef struct {
+    uint64_t seed;
+    int32_t n_ancestors;       /* e.g. 2000 */
+    int32_t desc_per_ancestor; /* e.g. 50 -> n_refs = n_ancestors * this */
+    int32_t anc_len_min, anc_len_max;
+    double div_min, div_max;   /* per-reference substitution rate range */
+    double indel_rate;         /* per-base indel probability (half ins, half del) */
+    int32_t n_reads;
+    int32_t read_len_min, read_len_max;
+    int32_t read_len_log_uniform; /* 0: uniform lengths, 1: log-uniform */
+    double err_min, err_max;   /* per-read substitution rate range */
+} hvk_gen_spec;
+
+/* total bases of the references / reads the spec produces */
+int hvk_gen_size(const hvk_gen_spec* spec, int64_t* ref_bases, int64_t* read_bases);
+/* ref_offsets: n_refs+1; read_offsets: n_reads+1; read_truth: source
+ * reference of each read (nullable). */
+int hvk_gen_fill(const hvk_gen_spec* spec, char* refs, int64_t* ref_offsets,
+                 char* reads, int64_t* read_offsets, int32_t* read_truth);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HDC_AMD_H */

@@ -1,0 +1,78 @@
+"""GPU encoder parity: hvk_encode vs the C oracle (HVCache::compute)."""
+import numpy as np
+import pytest
+
+import oracle_py as orc
+import seqgen
+
+pytestmark = pytest.mark.gpu
+
+CASES = [(10000, 5), (8192, 10), (10240, 12)]
+
+
+def _encode(dim, k, seqs, **kw):
+    import hdc_amd as H
+    import torch
+    ctx = H.Context(dim, k)
+    buf, off = H.concat(seqs)
+    dev = H.DeviceSeqs(buf, off)
+    enc = H.encode(ctx, dev, **kw)
+    torch.cuda.synchronize()
+    return ctx, enc
+
+
+@pytest.mark.parametrize("dim,k", CASES)
+def test_counts_bit_exact(dim, k):
+    seqs = seqgen.mixed_set(1000 + k, 24, 1, 400)
+    seqs += ["", "A" * (k - 1), "ACGT" * 3, "N" * 50, "acgtacgtacgtacgtacgt", "ACGTN" * 40]
+    print(f"encode D={dim} k={k} n={len(seqs)}", flush=True)
+    ctx, enc = _encode(dim, k, seqs)
+    counts = enc.counts.cpu().numpy()
+    lo = enc.lo.cpu().numpy().astype(np.int32)
+    hi = enc.hi.cpu().numpy().astype(np.int32)
+    sq = enc.sqnorm.cpu().numpy()
+    mx = enc.maxabs.cpu().numpy()
+    nk = enc.nkmers.cpu().numpy()
+    for i, s in enumerate(seqs):
+        ref, n = orc.compute(dim, k, s)
+        assert np.array_equal(counts[i, :dim], ref), f"seq {i} len {len(s)}"
+        assert not counts[i, dim:].any()
+        assert np.array_equal(256 * hi[i, :dim] + lo[i, :dim], ref.astype(np.int32))
+        assert sq[i] == int((ref.astype(np.int64) ** 2).sum())
+        assert mx[i] == int(np.abs(ref.astype(np.int32)).max())
+        assert nk[i] == n
+
+
+def test_long_sequences_planes():
+    """> 2048 k-mers (16 counter planes) and the int32 oracle sum."""
+    rng = np.random.default_rng(7)
+    seqs = [seqgen.random_seq(rng, 5000), "A" * 3000, seqgen.random_seq(rng, 2100)]
+    ctx, enc = _encode(10240, 12, seqs)
+    counts = enc.counts.cpu().numpy()
+    for i, s in enumerate(seqs):
+        ref32, _ = orc.compute32(10240, 12, s)
+        assert np.array_equal(counts[i, :10240].astype(np.int32), ref32)
+
+
+def test_generated_references_exact():
+    """Sample of the synthetic workload's references (1,300-1,600 nt)."""
+    import hdc_amd as H
+    spec = H.make_spec(n_ancestors=4, desc_per_ancestor=4, anc_len_min=1300, anc_len_max=1600,
+                       div_min=0.02, div_max=0.08, n_reads=16, read_len_min=150, read_len_max=300,
+                       err_min=0.005, err_max=0.02)
+    w = H.generate(spec)
+    refs = [w.refs[w.ref_offsets[i]:w.ref_offsets[i + 1]].decode() for i in range(16)]
+    reads = [w.reads[w.read_offsets[i]:w.read_offsets[i + 1]].decode() for i in range(16)]
+    ctx, enc = _encode(10240, 12, refs + reads)
+    counts = enc.counts.cpu().numpy()
+    for i, s in enumerate(refs + reads):
+        ref, _ = orc.compute(10240, 12, s)
+        assert np.array_equal(counts[i, :10240], ref), i
+
+
+def test_base_hv_matches_oracle():
+    import hdc_amd as H
+    for dim in (10000, 10240):
+        ctx = H.Context(dim, 5)
+        for ch in "ACGTN":
+            assert np.array_equal(ctx.base_hv(ch), orc.base_hv(dim, ch))

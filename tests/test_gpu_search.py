@@ -1,0 +1,158 @@
+"""GPU top-k search parity: plain (VALU) and MFMA forms vs the FP64
+brute-force oracle (order: cosine desc, reference index asc)."""
+import numpy as np
+import pytest
+
+import oracle_py as orc
+
+pytestmark = pytest.mark.gpu
+
+DIM, K = 10240, 12
+COS_REL_TOL = 1e-6  # relative tolerance of the float32 score vs FP64
+
+
+def encoded_from_counts(counts, ld):
+    """Encoded bundle set built on the host from an int16 count matrix."""
+    import hdc_amd as H
+    import torch
+    n, d = counts.shape
+    c = np.zeros((n, ld), np.int16)
+    c[:, :d] = counts
+    v = c.astype(np.int32)
+    lo = ((v + 128) % 256 - 128).astype(np.int8)
+    hi = ((v - lo) // 256).astype(np.int8)
+    sq = (c.astype(np.int64) ** 2).sum(1)
+    mx = np.abs(v).max(1).astype(np.int32) if n else np.zeros(0, np.int32)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    return H.Encoded(n, t(c), t(lo), t(hi), t(sq), t(mx), t(np.zeros(n, np.int32)))
+
+
+@pytest.fixture(scope="module")
+def workload():
+    import hdc_amd as H
+    import torch
+    spec = H.make_spec(n_ancestors=12, desc_per_ancestor=25, anc_len_min=1300, anc_len_max=1600,
+                       div_min=0.02, div_max=0.08, n_reads=157, read_len_min=150,
+                       read_len_max=300, err_min=0.005, err_max=0.02)
+    w = H.generate(spec)
+    ctx = H.Context(DIM, K)
+    refs = H.encode(ctx, H.DeviceSeqs(w.refs, w.ref_offsets))
+    reads = H.encode(ctx, H.DeviceSeqs(w.reads, w.read_offsets))
+    torch.cuda.synchronize()
+    rc = refs.counts.cpu().numpy()[:, :DIM]
+    qc = reads.counts.cpu().numpy()[:, :DIM]
+    return ctx, refs, reads, rc, qc, w
+
+
+def _check(res, ref_idx, ref_cos, k):
+    import torch
+    torch.cuda.synchronize()
+    idx = res.idx.cpu().numpy()
+    sc = res.score.cpu().numpy().astype(np.float64)
+    c64 = res.cos64.cpu().numpy()
+    assert np.array_equal(idx, ref_idx[:, :k])
+    ok = ref_idx[:, :k] >= 0
+    assert np.allclose(c64[ok], ref_cos[:, :k][ok], rtol=1e-12, atol=1e-15)
+    err = np.abs(sc[ok] - ref_cos[:, :k][ok]) / np.maximum(np.abs(ref_cos[:, :k][ok]), 1e-30)
+    assert err.max(initial=0.0) <= COS_REL_TOL
+
+
+@pytest.mark.parametrize("form", ["plain", "mfma"])
+@pytest.mark.parametrize("k", [1, 5, 10])
+def test_topk_matches_oracle(workload, form, k):
+    import hdc_amd as H
+    ctx, refs, reads, rc, qc, w = workload
+    print(f"search form={form} k={k} nq={reads.n} nr={refs.n}", flush=True)
+    ref_idx, ref_cos = orc.topk16(qc, rc, k)
+    res = (H.search_plain if form == "plain" else H.search_mfma)(ctx, reads, refs, k)
+    _check(res, ref_idx, ref_cos, k)
+
+
+@pytest.mark.parametrize("form", ["plain", "mfma"])
+def test_reads_find_their_source(workload, form):
+    """Top-1 of a read sampled from reference r is r (or an equal-score tie)."""
+    import hdc_amd as H
+    ctx, refs, reads, rc, qc, w = workload
+    res = (H.search_plain if form == "plain" else H.search_mfma)(ctx, reads, refs, 5)
+    idx = res.idx.cpu().numpy()
+    hit = (idx[:, 0] == w.truth).mean()
+    assert hit > 0.9
+
+
+@pytest.mark.parametrize("form", ["plain", "mfma"])
+def test_exact_ties_duplicate_references(workload, form):
+    """Duplicated references tie exactly; order must be index ascending and
+    the near-tie margin check must route the queries to the exact re-scan."""
+    import hdc_amd as H
+    ctx, refs, reads, rc, qc, w = workload
+    dup = np.concatenate([rc[:40], rc[:40], rc[5:6], rc[5:6], rc[5:6], rc[40:90]])
+    db = encoded_from_counts(dup, ctx.ld)
+    q = encoded_from_counts(qc[:33], ctx.ld)
+    ref_idx, ref_cos = orc.topk16(qc[:33], dup, 10)
+    res = (H.search_plain if form == "plain" else H.search_mfma)(ctx, q, db, 10)
+    _check(res, ref_idx, ref_cos, 10)
+    assert int(res.n_rescored.cpu().item()) > 0
+
+
+@pytest.mark.parametrize("form", ["plain", "mfma"])
+def test_zero_norm_rows(workload, form):
+    import hdc_amd as H
+    ctx, refs, reads, rc, qc, w = workload
+    db_np = rc[:70].copy()
+    db_np[3] = 0
+    db_np[64] = 0
+    q_np = qc[:5].copy()
+    q_np[2] = 0
+    db = encoded_from_counts(db_np, ctx.ld)
+    q = encoded_from_counts(q_np, ctx.ld)
+    ref_idx, ref_cos = orc.topk16(q_np, db_np, 5)
+    res = (H.search_plain if form == "plain" else H.search_mfma)(ctx, q, db, 5)
+    _check(res, ref_idx, ref_cos, 5)
+    assert list(res.idx.cpu().numpy()[2]) == [0, 1, 2, 3, 4]
+
+
+@pytest.mark.parametrize("form", ["plain", "mfma"])
+def test_k_larger_than_db(workload, form):
+    import hdc_amd as H
+    ctx, refs, reads, rc, qc, w = workload
+    db = encoded_from_counts(rc[:3], ctx.ld)
+    q = encoded_from_counts(qc[:4], ctx.ld)
+    res = (H.search_plain if form == "plain" else H.search_mfma)(ctx, q, db, 5)
+    idx = res.idx.cpu().numpy()
+    ref_idx, _ = orc.topk16(qc[:4], rc[:3], 5)
+    assert np.array_equal(idx, ref_idx)
+    assert (idx[:, 3:] == -1).all()
+
+
+def test_two_limb_queries(workload):
+    """Queries with |count| > 127 take the two-limb MFMA path."""
+    import hdc_amd as H
+    import torch
+    ctx, refs, reads, rc, qc, w = workload
+    seqs = [w.refs[w.ref_offsets[i]:w.ref_offsets[i + 1]].decode() for i in (0, 30, 77)]
+    seqs += ["ACGTTGCA" * 150, "A" * 800]
+    buf, off = H.concat(seqs)
+    q = H.encode(ctx, H.DeviceSeqs(buf, off))
+    torch.cuda.synchronize()
+    assert int(q.maxabs.max().item()) > 127
+    qn = q.counts.cpu().numpy()[:, :DIM]
+    ref_idx, ref_cos = orc.topk16(qn, rc, 5)
+    res = H.search_mfma(ctx, q, refs, 5)
+    _check(res, ref_idx, ref_cos, 5)
+    assert list(res.idx.cpu().numpy()[0][:1]) == [0]
+
+
+def test_bipolar_single_limb():
+    """Binarised bipolar HVs (+1/-1) as one int8 limb on both sides."""
+    import hdc_amd as H
+    rng = np.random.default_rng(3)
+    ctx = H.Context(DIM, K)
+    db_np = np.where(rng.random((300, DIM)) < 0.5, 1, -1).astype(np.int16)
+    q_np = db_np[rng.integers(0, 300, 64)].copy()
+    flip = rng.random(q_np.shape) < 0.2
+    q_np[flip] *= -1
+    db = encoded_from_counts(db_np, ctx.ld)
+    q = encoded_from_counts(q_np, ctx.ld)
+    ref_idx, ref_cos = orc.topk16(q_np, db_np, 5)
+    res = H.search_mfma(ctx, q, db, 5, q_two_limbs=False, r_two_limbs=False)
+    _check(res, ref_idx, ref_cos, 5)
