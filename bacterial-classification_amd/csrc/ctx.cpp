@@ -74,6 +74,7 @@ int hvk_ctx_create(int dim, int kmer, hvk_ctx** out) {
     c->ld = hvk::row_ld_for(dim);
     c->device = dev;
     c->n_cu = prop.multiProcessorCount;
+    c->max_lds = (int)prop.sharedMemPerBlock;
     const int excess = c->w64 * 64 - dim;
     const uint64_t tail = excess ? (~0ULL) >> excess : ~0ULL;
     const int seeds[4] = {0, 1, 3, 2};  // A C G T

@@ -241,12 +241,7 @@ int launch_kp(hvk_ctx* ctx, const uint32_t* packed, const uint32_t* invalid,
     const int W32 = ctx->w32;
     const int threads = (W32 + 63) / 64 * 64;
     const size_t lds = sizeof(uint32_t) * ((size_t)4 * ctx->k * W32 + kChunk);
-    static bool attr_set = false;
-    if (!attr_set) {
-        HVK_HIP(hipFuncSetAttribute((const void*)encode_kernel<K, P>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        attr_set = true;
-    }
+    if (lds + 256 > (size_t)ctx->max_lds) return HVK_ERR_ARG;
     int grid = n < 4 * ctx->n_cu ? n : 4 * ctx->n_cu;
     timing_mark(ctx, 0, false, st);
     hipLaunchKernelGGL((encode_kernel<K, P>), dim3(grid), dim3(threads), lds, st, ctx->d_table,
@@ -295,8 +290,6 @@ extern "C" int hvk_encode(hvk_ctx* ctx, const uint32_t* d_packed, const uint32_t
     if (ld == 0) ld = ctx->ld;
     if (ld < 32 * (int64_t)ctx->w32 || (ld % 16) != 0) return HVK_ERR_ARG;
     if ((int64_t)max_len - ctx->k + 1 >= (1 << 24)) return HVK_ERR_RANGE;
-    // LDS budget: rotated table + one chunk of codes
-    if ((size_t)4 * ctx->k * ctx->w32 * 4 + 4096 > 150 * 1024) return HVK_ERR_ARG;
     return hvk::launch_encode(ctx, d_packed, d_invalid, d_offsets, n_seqs, max_len, d_counts,
                               d_lo, d_hi, ld, d_sqnorm, d_maxabs, d_nkmers,
                               (hipStream_t)stream);

@@ -49,6 +49,7 @@ struct hvk_ctx {
     int64_t ld = 0;
     int device = 0;
     int n_cu = 0;
+    int max_lds = 65536;  // bytes of LDS one workgroup may use
     std::vector<uint64_t> base[4];      // A C G T (code order), W words each
     uint32_t* d_table = nullptr;        // [4][k][w32] rotated base words
     hvk::Workspace ws_search;           // partial top lists

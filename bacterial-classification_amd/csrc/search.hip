@@ -3,19 +3,19 @@
 // Order: (c desc, reference index asc), c = dot / (sqrt(|q|^2) sqrt(|r|^2))
 // in FP64 from the exact integer dot (the reference's cosine, src/HV.cpp:
 // 219-242, before its 2c-1 map).  Pipeline per call:
-//   1. inv_norm_kernel   FP32 inverse norms 1/sqrt(|v|^2) (correctly rounded)
+//   1. inv_norm_kernel  FP32 inverse norms 1/sqrt(|v|^2), correctly rounded.
 //   2. plain_kernel | mfma_kernel
-//        tile of MT queries x NT references, K-slabs staged through LDS; the
-//        tile's exact int64 dots are scored in FP32, s = dot * iq * ir, and
-//        offered to per-query LDS top lists keyed by (s, -index).  Each entry
-//        carries its exact dot.  A workgroup sweeps a slice ("split") of the
-//        references and writes its per-query top-KP list.
-//   3. finalize_kernel   merges the split lists, re-scores the KP survivors
-//        in FP64 from their exact dots, sorts, and checks the margin: every
-//        excluded reference has s <= s_KP, so |s - c| <= 8 ulp guarantees
-//        exactness when c_k > s_KP + 1e-6.  Queries failing the check are
-//        queued for
-//   4. rescan_kernel     an exact FP64 brute-force pass over all references.
+//        MT queries x NT references per tile, K-slabs staged through LDS.
+//        Each tile's scores s = dot * iq * ir (FP32) are offered to per-query
+//        LDS top lists keyed by (s, -index).  A workgroup sweeps one slice
+//        ("split") of the references and writes its per-query top-KP keys.
+//   3. finalize_kernel  one wave per query: merges the split lists,
+//        recomputes the exact int64 dot of each of the KP survivors from the
+//        stored HVs, re-scores them in FP64, sorts, and checks the margin:
+//        every excluded reference has s <= s_KP and |s - c| < kMarginEps, so
+//        the top-k is exact when c_k > s_KP + kMarginEps.  Queries failing
+//        the check are queued for
+//   4. rescan_kernel    an exact FP64 brute-force pass over all references.
 #include <cmath>
 #include <utility>
 
@@ -28,12 +28,9 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 typedef short v2s __attribute__((ext_vector_type(2)));
 
-constexpr float kMarginEps = 1e-6f;
-
-struct Cand {
-    uint64_t key;  // (ord(s) << 32) | (~index); 0 = empty
-    int64_t dot;
-};
+// Bound on |s - c| for the FP32 score: below 16 units in the last place of
+// 1.0 (2^-24) for the worst limb combination (two limbs x two limbs).
+constexpr float kMarginEps = 2e-6f;
 
 __device__ __forceinline__ uint32_t ord_f32(float f) {
     const uint32_t b = __float_as_uint(f);
@@ -55,55 +52,6 @@ __device__ __forceinline__ double cos64(int64_t dot, int64_t nq, int64_t nr) {
     return (double)dot / (sqrt((double)nq) * sqrt((double)nr));
 }
 
-// ------------------------------------------------------------------------
-// per-workgroup top lists in LDS
-// ------------------------------------------------------------------------
-template <int MT, int KP, int CAP>
-struct TopLds {
-    Cand top[MT][KP];  // descending by key
-    Cand buf[MT][CAP];
-    uint64_t thr[MT];  // top[m][KP-1].key
-    int cnt[MT];
-    int ovf[2];
-
-    __device__ void init(int tid, int nthr) {
-        for (int i = tid; i < MT * KP; i += nthr) top[i / KP][i % KP] = Cand{0, 0};
-        for (int i = tid; i < MT; i += nthr) { thr[i] = 0; cnt[i] = 0; }
-        if (tid == 0) { ovf[0] = 0; ovf[1] = 0; }
-    }
-
-    // Offers one (query m, reference) pair.  Returns true when the pair beat
-    // the threshold but the buffer was full: the caller keeps it for the next
-    // pass, which runs after the merge has raised the threshold.  A pair is
-    // therefore offered at most once until it is accepted or falls below.
-    __device__ __forceinline__ bool offer(int m, uint64_t key, int64_t dot, int pass) {
-        if (key <= thr[m]) return false;
-        const int slot = atomicAdd(&cnt[m], 1);
-        if (slot < CAP) {
-            buf[m][slot] = Cand{key, dot};
-            return false;
-        }
-        ovf[pass & 1] = 1;
-        return true;
-    }
-
-    __device__ void merge(int m) {
-        const int n = cnt[m] < CAP ? cnt[m] : CAP;
-        for (int q = 0; q < n; ++q) {
-            const Cand c = buf[m][q];
-            if (c.key <= top[m][KP - 1].key) continue;
-            int p = KP - 1;
-            while (p > 0 && top[m][p - 1].key < c.key) {
-                top[m][p] = top[m][p - 1];
-                --p;
-            }
-            top[m][p] = c;
-        }
-        thr[m] = top[m][KP - 1].key;
-        cnt[m] = 0;
-    }
-};
-
 template <typename F, int... Is>
 __device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, Is...>) {
     (f(std::integral_constant<int, Is>{}), ...);
@@ -115,8 +63,59 @@ __device__ __forceinline__ void static_for(F&& f) {
     static_for_impl(f, std::make_integer_sequence<int, N>{});
 }
 
-// End of one offer pass: merge the buffers, report whether any pair was
-// dropped (then the caller runs another pass over its dropped pairs).
+// ------------------------------------------------------------------------
+// per-workgroup top lists in LDS (keys only; 0 = empty)
+// ------------------------------------------------------------------------
+template <int MT, int KP, int CAP>
+struct TopLds {
+    uint64_t top[MT][KP];  // descending
+    uint64_t buf[MT][CAP];
+    uint64_t thr[MT];      // top[m][KP-1]
+    float thr_f[MT];       // its FP32 score, -inf while the list is not full
+    int cnt[MT];
+    int ovf[2];
+
+    __device__ void init(int tid, int nthr) {
+        for (int i = tid; i < MT * KP; i += nthr) top[i / KP][i % KP] = 0;
+        for (int i = tid; i < MT; i += nthr) { thr[i] = 0; thr_f[i] = -INFINITY; cnt[i] = 0; }
+        if (tid == 0) { ovf[0] = 0; ovf[1] = 0; }
+    }
+
+    // Offers one (query m, reference) key.  Returns true when it beat the
+    // threshold but the buffer was full: the caller keeps it for the next
+    // pass, which runs after the merge has raised the threshold.  A key is
+    // therefore offered at most once until accepted or fallen below.
+    __device__ __forceinline__ bool offer(int m, uint64_t key, int pass) {
+        if (key <= thr[m]) return false;
+        const int slot = atomicAdd(&cnt[m], 1);
+        if (slot < CAP) {
+            buf[m][slot] = key;
+            return false;
+        }
+        ovf[pass & 1] = 1;
+        return true;
+    }
+
+    __device__ void merge(int m) {
+        const int n = cnt[m] < CAP ? cnt[m] : CAP;
+        for (int q = 0; q < n; ++q) {
+            const uint64_t c = buf[m][q];
+            if (c <= top[m][KP - 1]) continue;
+            int p = KP - 1;
+            while (p > 0 && top[m][p - 1] < c) {
+                top[m][p] = top[m][p - 1];
+                --p;
+            }
+            top[m][p] = c;
+        }
+        thr[m] = top[m][KP - 1];
+        thr_f[m] = thr[m] ? f32_of_ord((uint32_t)(thr[m] >> 32)) : -INFINITY;
+        cnt[m] = 0;
+    }
+};
+
+// End of one offer pass: merge the buffers; true when a key was dropped
+// (the caller then runs another pass over its dropped keys).
 template <int MT, int KP, int CAP>
 __device__ __forceinline__ bool topk_pass_end(TopLds<MT, KP, CAP>& tl, int tid, int pass) {
     __syncthreads();
@@ -155,7 +154,7 @@ template <int KP>
 __global__ void __launch_bounds__(256)
 plain_kernel(const int16_t* __restrict__ q, const float* __restrict__ iq, int nq,
              const int16_t* __restrict__ r, const float* __restrict__ ir, int nr, int64_t ld,
-             int n_splits, int split_tiles, Cand* __restrict__ part) {
+             int split_tiles, uint64_t* __restrict__ part) {
     constexpr int MT = kPlainMT, NT = kPlainNT, KS = kPlainKS, KW = KS / 2;
     constexpr int CAP = 16;
     __shared__ uint32_t sQ[KW][MT];
@@ -247,7 +246,7 @@ plain_kernel(const int16_t* __restrict__ q, const float* __restrict__ iq, int nq
                 const int n = n0 + tx + 32 * j;
                 if ((pass == 0 || (dropped & bit)) && m0 + m < nq && n < nr) {
                     const float s = __fmul_rn(__fmul_rn((float)acc[i][j], s_iq[m]), irv[j]);
-                    if (tl.offer(m, make_key(s, n), acc[i][j], pass)) dropped |= bit;
+                    if (tl.offer(m, make_key(s, n), pass)) dropped |= bit;
                     else dropped &= ~bit;
                 }
             });
@@ -261,103 +260,156 @@ plain_kernel(const int16_t* __restrict__ q, const float* __restrict__ iq, int nq
 }
 
 // ------------------------------------------------------------------------
-// 2b. MFMA form: v_mfma_i32_32x32x32_i8, 128 x 128 tile, 4 waves (2 x 2),
-//     wave tile 64 x 64, K-slab of 64 bytes, XOR-swizzled LDS rows.
+// 2b. MFMA form: v_mfma_i32_32x32x32_i8.  Workgroup of 4 waves (one per
+//     SIMD, 512 registers each) on an MT-query x 128-reference tile; wave
+//     grid 2 (queries) x 2 (references), wave tile (32 WX) x 64, i.e. WX x 2
+//     MFMA blocks per limb pair with the accumulators in AGPRs.
+//     MT = 256 (WX = 4) for one query limb, 128 (WX = 2) for two.
+//     K-slabs of 64 bytes, double-buffered in LDS (rows XOR-swizzled in
+//     16-byte chunks), the next slab prefetched into registers while the
+//     current one is multiplied: one barrier per slab.
 //     Queries are the A operand (rows), references the B operand (columns).
 // ------------------------------------------------------------------------
-constexpr int kMfmaMT = 128, kMfmaNT = 128, kMfmaKS = 64;
+constexpr int kMfmaNT = 128, kMfmaKS = 64, kMfmaThreads = 256;
+template <int QL> constexpr int mfma_wx() { return 2; }
+template <int QL> constexpr int mfma_mt() { return 64 * mfma_wx<QL>(); }
+
+// 16-byte global -> LDS copy (global_load_lds_dwordx4): lane l of the wave
+// writes LDS bytes [lds + 16 l, lds + 16 l + 16); lds must be wave-uniform.
+// The builtin exists only for the device pass.
+__device__ __forceinline__ void load_lds16(const int8_t* g, int8_t* lds) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __builtin_amdgcn_global_load_lds(g, lds, 16, 0, 0);
+#endif
+}
 
 __device__ __forceinline__ int swz(int row, int chunk) {
     return row * 64 + ((chunk ^ ((row >> 1) & 3)) << 4);
 }
 
 template <int QL, int RL, int KP>
-__global__ void __launch_bounds__(256, 1)
+struct MfmaSmem {
+    int8_t q[2][QL][mfma_mt<QL>() * kMfmaKS];
+    int8_t r[2][RL][kMfmaNT * kMfmaKS];
+    TopLds<mfma_mt<QL>(), KP, 8> tl;
+    float iq[mfma_mt<QL>()];
+};
+
+template <int QL, int RL, int KP>
+__global__ void __launch_bounds__(kMfmaThreads, 2)
 mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
             const float* __restrict__ iq, int nq, const int8_t* __restrict__ rlo,
             const int8_t* __restrict__ rhi, const float* __restrict__ ir, int nr, int64_t ld,
-            int n_splits, int split_tiles, Cand* __restrict__ part) {
-    constexpr int MT = kMfmaMT, NT = kMfmaNT, KS = kMfmaKS;
-    constexpr int CAP = 16;
-    __shared__ __attribute__((aligned(16))) int8_t sQ[QL][MT * KS];
-    __shared__ __attribute__((aligned(16))) int8_t sR[RL][NT * KS];
-    __shared__ TopLds<MT, KP, CAP> tl;
-    __shared__ float s_iq[MT];
+            int split_tiles, uint64_t* __restrict__ part) {
+    constexpr int WX = mfma_wx<QL>(), MT = mfma_mt<QL>(), NT = kMfmaNT, KS = kMfmaKS;
+    constexpr int NTH = kMfmaThreads;
+    constexpr int NPAIR = WX * 2 * 16;       // (query, reference) pairs per lane
+    __shared__ __attribute__((aligned(16))) MfmaSmem<QL, RL, KP> sm;
+    auto& tl = sm.tl;
 
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     const int m0 = blockIdx.x * MT;
     const int split = blockIdx.y;
-    tl.init(tid, 256);
-    if (tid < MT) s_iq[tid] = (m0 + tid < nq) ? iq[m0 + tid] : 0.0f;
+    tl.init(tid, NTH);
+    if (tid < MT) sm.iq[tid] = (m0 + tid < nq) ? iq[m0 + tid] : 0.0f;
     const int n_begin = split * split_tiles * NT;
     int n_end = n_begin + split_tiles * NT;
     if (n_end > nr) n_end = nr;
     const int8_t* qsrc[2] = {qlo, qhi};
     const int8_t* rsrc[2] = {rlo, rhi};
+    const int nk = (int)(ld / KS);
+
+    // Direct global->LDS loads: one wave instruction fills 1 KB of LDS (16
+    // rows of 64 B), lane l writing bytes [16 l, 16 l + 16).  Lane l of
+    // wave-load j therefore fetches row 16 j + l / 4, logical chunk
+    // (l & 3) ^ swizzle(row), so that LDS holds the swizzled layout swz().
+    constexpr int QJ = MT * KS / 1024 / 4;  // wave-loads per wave per Q limb
+    constexpr int RJ = NT * KS / 1024 / 4;  // per R limb
+    int q_off[QJ], r_off[RJ];               // byte offset of the lane's chunk in its row
+    int q_rowi[QJ], r_rowi[RJ];             // row within the tile
+#pragma unroll
+    for (int t = 0; t < QJ; ++t) {
+        const int j = wave * QJ + t, row = 16 * j + (lane >> 2);
+        q_rowi[t] = row;
+        q_off[t] = 16 * ((lane & 3) ^ ((row >> 1) & 3));
+    }
+#pragma unroll
+    for (int t = 0; t < RJ; ++t) {
+        const int j = wave * RJ + t, row = 16 * j + (lane >> 2);
+        r_rowi[t] = row;
+        r_off[t] = 16 * ((lane & 3) ^ ((row >> 1) & 3));
+    }
     __syncthreads();
 
     for (int n0 = n_begin; n0 < n_end; n0 += NT) {
-        v16i acc[QL][RL][2][2];
+        // rows past the end read a valid row; their results are never offered
+        const int8_t* gq[QL][QJ];
+        const int8_t* gr[RL][RJ];
 #pragma unroll
-        for (int a = 0; a < QL; ++a)
+        for (int l = 0; l < QL; ++l)
 #pragma unroll
-            for (int b = 0; b < RL; ++b)
+            for (int t = 0; t < QJ; ++t) {
+                const int row = min(m0 + q_rowi[t], nq - 1);
+                gq[l][t] = qsrc[l] + (int64_t)row * ld + q_off[t];
+            }
 #pragma unroll
-                for (int x = 0; x < 2; ++x)
+        for (int l = 0; l < RL; ++l)
 #pragma unroll
-                    for (int y = 0; y < 2; ++y) acc[a][b][x][y] = v16i{};
-
-        for (int64_t k0 = 0; k0 < ld; k0 += KS) {
+            for (int t = 0; t < RJ; ++t) {
+                const int row = min(n0 + r_rowi[t], nr - 1);
+                gr[l][t] = rsrc[l] + (int64_t)row * ld + r_off[t];
+            }
+        auto issue_slab = [&](int kk, int buf) {
+            const int64_t k0 = (int64_t)kk * KS;
 #pragma unroll
             for (int l = 0; l < QL; ++l)
 #pragma unroll
-                for (int it = 0; it < 2; ++it) {
-                    const int c = it * 256 + tid;
-                    const int row = c >> 2, ch = c & 3;
-                    v4i v = v4i{};
-                    if (m0 + row < nq)
-                        v = *reinterpret_cast<const v4i*>(qsrc[l] + (int64_t)(m0 + row) * ld + k0 + ch * 16);
-                    *reinterpret_cast<v4i*>(&sQ[l][swz(row, ch)]) = v;
-                }
+                for (int t = 0; t < QJ; ++t)
+                    load_lds16(gq[l][t] + k0, &sm.q[buf][l][1024 * (wave * QJ + t)]);
 #pragma unroll
             for (int l = 0; l < RL; ++l)
 #pragma unroll
-                for (int it = 0; it < 2; ++it) {
-                    const int c = it * 256 + tid;
-                    const int row = c >> 2, ch = c & 3;
-                    v4i v = v4i{};
-                    if (n0 + row < nr)
-                        v = *reinterpret_cast<const v4i*>(rsrc[l] + (int64_t)(n0 + row) * ld + k0 + ch * 16);
-                    *reinterpret_cast<v4i*>(&sR[l][swz(row, ch)]) = v;
-                }
-            __syncthreads();
+                for (int t = 0; t < RJ; ++t)
+                    load_lds16(gr[l][t] + k0, &sm.r[buf][l][1024 * (wave * RJ + t)]);
+        };
+
+        v16i acc[QL][RL][WX][2];
+        static_for<QL * RL * WX * 2>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            acc[i / (RL * WX * 2)][(i / (WX * 2)) % RL][(i / 2) % WX][i % 2] = v16i{};
+        });
+
+        issue_slab(0, 0);
+        __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+        __syncthreads();
+        for (int kk = 0; kk < nk; ++kk) {
+            const int cur = kk & 1;
+            if (kk + 1 < nk) issue_slab(kk + 1, cur ^ 1);
 #pragma unroll
             for (int ks = 0; ks < KS / 32; ++ks) {
                 const int ch = ks * 2 + (lane >> 5);
-                v4i a[QL][2], b[RL][2];
+                v4i a[QL][WX], b[RL][2];
 #pragma unroll
                 for (int l = 0; l < QL; ++l)
 #pragma unroll
-                    for (int x = 0; x < 2; ++x)
-                        a[l][x] = *reinterpret_cast<const v4i*>(&sQ[l][swz(wm * 64 + x * 32 + (lane & 31), ch)]);
+                    for (int x = 0; x < WX; ++x)
+                        a[l][x] = *reinterpret_cast<const v4i*>(&sm.q[cur][l][swz(wm * 32 * WX + x * 32 + (lane & 31), ch)]);
 #pragma unroll
                 for (int l = 0; l < RL; ++l)
 #pragma unroll
                     for (int y = 0; y < 2; ++y)
-                        b[l][y] = *reinterpret_cast<const v4i*>(&sR[l][swz(wn * 64 + y * 32 + (lane & 31), ch)]);
-#pragma unroll
-                for (int qa = 0; qa < QL; ++qa)
-#pragma unroll
-                    for (int rb = 0; rb < RL; ++rb)
-#pragma unroll
-                        for (int x = 0; x < 2; ++x)
-#pragma unroll
-                            for (int y = 0; y < 2; ++y)
-                                acc[qa][rb][x][y] = __builtin_amdgcn_mfma_i32_32x32x32_i8(
-                                    a[qa][x], b[rb][y], acc[qa][rb][x][y], 0, 0, 0);
+                        b[l][y] = *reinterpret_cast<const v4i*>(&sm.r[cur][l][swz(wn * 64 + y * 32 + (lane & 31), ch)]);
+                static_for<QL * RL * WX * 2>([&](auto I) {
+                    constexpr int i = decltype(I)::value;
+                    constexpr int qa = i / (RL * WX * 2), rb = (i / (WX * 2)) % RL;
+                    constexpr int x = (i / 2) % WX, y = i % 2;
+                    acc[qa][rb][x][y] = __builtin_amdgcn_mfma_i32_32x32x32_i8(
+                        a[qa][x], b[rb][y], acc[qa][rb][x][y], 0, 0, 0);
+                });
             }
+            __builtin_amdgcn_s_waitcnt(0x0F70);  // this wave's loads of slab kk+1 landed
             __syncthreads();
         }
 
@@ -367,70 +419,169 @@ mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
             const int n = n0 + wn * 64 + y * 32 + (lane & 31);
             irv[y] = n < nr ? ir[n] : 0.0f;
         }
-        uint64_t dropped = 0;  // bit 32*x+16*y+e: pair waits for another pass
+        // Epilogue.  Pair p = (x * 2 + y) * 16 + e is query row
+        // wm*32*WX + x*32 + 8(e/4) + 4(lane/32) + e%4, reference column
+        // n0 + wn*64 + y*32 + lane%32 (the 32x32 MFMA output layout).  A float
+        // prefilter against the query's current FP32 threshold rejects almost
+        // every pair; only survivors build keys and enter the LDS lists.
+        uint64_t dropped[(NPAIR + 63) / 64];
+#pragma unroll
+        for (int i = 0; i < (NPAIR + 63) / 64; ++i) dropped[i] = 0;
         for (int pass = 0;; ++pass) {
-            static_for<64>([&](auto I) {
-                constexpr int x = decltype(I)::value / 32, y = (decltype(I)::value / 16) % 2,
-                              e = decltype(I)::value % 16;
-                constexpr uint64_t bit = 1ull << decltype(I)::value;
+            static_for<WX * 2>([&](auto XY) {
+                constexpr int x = decltype(XY)::value / 2, y = decltype(XY)::value % 2;
                 const int n = n0 + wn * 64 + y * 32 + (lane & 31);
-                const int m = wm * 64 + x * 32 + 8 * (e >> 2) + 4 * (lane >> 5) + (e & 3);
-                if ((pass == 0 || (dropped & bit)) && n < nr && m0 + m < nq) {
-                    int64_t dot = 0;
-                    static_for<QL>([&](auto QA) {
-                        static_for<RL>([&](auto RB) {
-                            constexpr int qa = decltype(QA)::value, rb = decltype(RB)::value;
-                            dot += (int64_t)acc[qa][rb][x][y][e] << (8 * (qa + rb));
+                const int mb = wm * 32 * WX + x * 32 + 4 * (lane >> 5);
+                float sv[16];
+                bool hit = false;
+                static_for<4>([&](auto G) {
+                    constexpr int g = decltype(G)::value;
+                    const float4 iq4 = *reinterpret_cast<const float4*>(&sm.iq[mb + 8 * g]);
+                    const float4 th4 = *reinterpret_cast<const float4*>(&tl.thr_f[mb + 8 * g]);
+                    const float iqv[4] = {iq4.x, iq4.y, iq4.z, iq4.w};
+                    const float thv[4] = {th4.x, th4.y, th4.z, th4.w};
+                    static_for<4>([&](auto H) {
+                        constexpr int h = decltype(H)::value, e = 4 * g + h;
+                        constexpr int p = (x * 2 + y) * 16 + e;
+                        float v = 0.0f;  // sum_{qa,rb} acc * 256^(qa+rb) in FP32
+                        static_for<QL * RL>([&](auto J) {
+                            constexpr int qa = decltype(J)::value / RL, rb = decltype(J)::value % RL;
+                            constexpr float scale = (float)(1u << (8 * (qa + rb)));
+                            v = __fmaf_rn((float)acc[qa][rb][x][y][e], scale, v);
                         });
+                        sv[e] = __fmul_rn(__fmul_rn(v, iqv[h]), irv[y]);
+                        const bool want = pass == 0 || ((dropped[p / 64] >> (p % 64)) & 1);
+                        hit |= want && sv[e] >= thv[h];
                     });
-                    const float s = __fmul_rn(__fmul_rn((float)dot, s_iq[m]), irv[y]);
-                    if (tl.offer(m, make_key(s, n), dot, pass)) dropped |= bit;
-                    else dropped &= ~bit;
+                });
+                if (hit && n < nr) {
+                    static_for<16>([&](auto E) {
+                        constexpr int e = decltype(E)::value, p = (x * 2 + y) * 16 + e;
+                        constexpr uint64_t bit = 1ull << (p % 64);
+                        const int m = mb + 8 * (e >> 2) + (e & 3);
+                        const bool want = pass == 0 || (dropped[p / 64] & bit);
+                        if (want && m0 + m < nq) {
+                            if (tl.offer(m, make_key(sv[e], n), pass)) dropped[p / 64] |= bit;
+                            else dropped[p / 64] &= ~bit;
+                        }
+                    });
                 }
             });
             if (!topk_pass_end(tl, tid, pass)) break;
         }
     }
-    for (int i = tid; i < MT * KP; i += 256) {
+    for (int i = tid; i < MT * KP; i += NTH) {
         const int m = i / KP, j = i % KP;
         if (m0 + m < nq) part[((int64_t)split * nq + m0 + m) * KP + j] = tl.top[m][j];
     }
 }
 
 // ------------------------------------------------------------------------
-// 3. finalize: merge splits, FP64 re-score of the KP survivors, margin check
+// Exact int64 dot of two stored rows by one wave: int16 counts (plain form)
+// or int8 limbs (value = lo + 256 hi; hi rows may be absent).  Every lane
+// returns the full dot.
 // ------------------------------------------------------------------------
-template <int KP>
-__global__ void __launch_bounds__(128)
-finalize_kernel(const Cand* __restrict__ part, int n_splits, int nq,
+__device__ __forceinline__ int64_t wave_sum64(int64_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const int lo = __shfl_xor((int)(uint32_t)v, o, 64);
+        const int hi = __shfl_xor((int)(v >> 32), o, 64);
+        v += (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+    }
+    return v;
+}
+
+struct RowsI16 {
+    const int16_t* q;
+    const int16_t* r;
+    int64_t ld;
+    __device__ int64_t dot(int lane, int64_t qrow, int64_t rrow) const {
+        const uint4* a = reinterpret_cast<const uint4*>(q + qrow * ld);
+        const uint4* b = reinterpret_cast<const uint4*>(r + rrow * ld);
+        int64_t s = 0;
+        for (int64_t c = lane; c < ld / 8; c += 64) {
+            const uint4 x = a[c], y = b[c];
+            int t = 0;
+            t = __builtin_amdgcn_sdot2(__builtin_bit_cast(v2s, x.x), __builtin_bit_cast(v2s, y.x), t, false);
+            t = __builtin_amdgcn_sdot2(__builtin_bit_cast(v2s, x.y), __builtin_bit_cast(v2s, y.y), t, false);
+            t = __builtin_amdgcn_sdot2(__builtin_bit_cast(v2s, x.z), __builtin_bit_cast(v2s, y.z), t, false);
+            t = __builtin_amdgcn_sdot2(__builtin_bit_cast(v2s, x.w), __builtin_bit_cast(v2s, y.w), t, false);
+            s += t;
+        }
+        return wave_sum64(s);
+    }
+};
+
+__device__ __forceinline__ int dot4x4(const uint4& x, const uint4& y) {
+    int t = __builtin_amdgcn_sdot4((int)x.x, (int)y.x, 0, false);
+    t = __builtin_amdgcn_sdot4((int)x.y, (int)y.y, t, false);
+    t = __builtin_amdgcn_sdot4((int)x.z, (int)y.z, t, false);
+    return __builtin_amdgcn_sdot4((int)x.w, (int)y.w, t, false);
+}
+
+struct RowsLimb {
+    const int8_t *qlo, *qhi, *rlo, *rhi;
+    int64_t ld;
+    __device__ int64_t dot(int lane, int64_t qrow, int64_t rrow) const {
+        int64_t s = 0;
+        for (int64_t c = lane; c < ld / 16; c += 64) {
+            const int64_t oq = qrow * ld + 16 * c, orr = rrow * ld + 16 * c;
+            const uint4 ql = *reinterpret_cast<const uint4*>(qlo + oq);
+            const uint4 rl = *reinterpret_cast<const uint4*>(rlo + orr);
+            int64_t t = dot4x4(ql, rl);
+            if (rhi) t += (int64_t)dot4x4(ql, *reinterpret_cast<const uint4*>(rhi + orr)) << 8;
+            if (qhi) {
+                const uint4 qh = *reinterpret_cast<const uint4*>(qhi + oq);
+                t += (int64_t)dot4x4(qh, rl) << 8;
+                if (rhi) t += (int64_t)dot4x4(qh, *reinterpret_cast<const uint4*>(rhi + orr)) << 16;
+            }
+            s += t;
+        }
+        return wave_sum64(s);
+    }
+};
+
+// ------------------------------------------------------------------------
+// 3. finalize: one wave per query
+// ------------------------------------------------------------------------
+template <int KP, typename Rows>
+__global__ void __launch_bounds__(256)
+finalize_kernel(Rows rows, const uint64_t* __restrict__ part, int n_splits, int nq,
                 const int64_t* __restrict__ qn, const int64_t* __restrict__ rn, int nr, int k,
                 int32_t* __restrict__ out_idx, float* __restrict__ out_score,
                 double* __restrict__ out_cos, int32_t* __restrict__ flag_list,
                 int32_t* __restrict__ flag_count) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m >= nq) return;
-    Cand best[KP];
-#pragma unroll
-    for (int j = 0; j < KP; ++j) best[j] = part[(int64_t)m * KP + j];
-    for (int s = 1; s < n_splits; ++s) {
-        for (int j = 0; j < KP; ++j) {
-            const Cand c = part[((int64_t)s * nq + m) * KP + j];
-            if (c.key <= best[KP - 1].key) break;  // lists are sorted
-            int p = KP - 1;
-            while (p > 0 && best[p - 1].key < c.key) { best[p] = best[p - 1]; --p; }
-            best[p] = c;
-        }
+    __shared__ uint64_t s_key[4][KP];
+    __shared__ double s_c[4][KP];
+    __shared__ int s_i[4][KP];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int m = blockIdx.x * 4 + wv;
+    if (m >= nq) return;  // whole wave exits together
+    uint64_t* key = s_key[wv];
+    if (lane == 0) {
+        for (int j = 0; j < KP; ++j) key[j] = part[(int64_t)m * KP + j];
+        for (int s = 1; s < n_splits; ++s)
+            for (int j = 0; j < KP; ++j) {
+                const uint64_t c = part[((int64_t)s * nq + m) * KP + j];
+                if (c <= key[KP - 1]) break;  // each list is sorted
+                int p = KP - 1;
+                while (p > 0 && key[p - 1] < c) { key[p] = key[p - 1]; --p; }
+                key[p] = c;
+            }
     }
-    double cs[KP];
-    int ix[KP];
+    __builtin_amdgcn_wave_barrier();
     int nv = 0;
+    while (nv < KP && key[nv] != 0) ++nv;
     const int64_t qnm = qn[m];
-    for (int j = 0; j < KP; ++j) {
-        if (best[j].key == 0) break;
-        ix[nv] = idx_of_key(best[j].key);
-        cs[nv] = cos64(best[j].dot, qnm, rn[ix[nv]]);
-        ++nv;
+    for (int j = 0; j < nv; ++j) {
+        const int idx = idx_of_key(key[j]);
+        const int64_t d = rows.dot(lane, m, idx);
+        if (lane == 0) { s_i[wv][j] = idx; s_c[wv][j] = cos64(d, qnm, rn[idx]); }
     }
+    __builtin_amdgcn_wave_barrier();
+    if (lane != 0) return;
+    double* cs = s_c[wv];
+    int* ix = s_i[wv];
     for (int a = 1; a < nv; ++a) {  // insertion sort: c desc, index asc
         const double c = cs[a];
         const int i = ix[a];
@@ -445,7 +596,7 @@ finalize_kernel(const Cand* __restrict__ part, int n_splits, int nq,
     }
     bool flag = false;
     if (nv == KP && nr > KP) {
-        const float s_kp = f32_of_ord((uint32_t)(best[KP - 1].key >> 32));
+        const float s_kp = f32_of_ord((uint32_t)(key[KP - 1] >> 32));
         flag = !(cs[k - 1] > (double)s_kp + (double)kMarginEps);
     }
     for (int j = 0; j < k; ++j) {
@@ -464,92 +615,66 @@ finalize_kernel(const Cand* __restrict__ part, int n_splits, int nq,
 }
 
 // ------------------------------------------------------------------------
-// 4. exact re-scan of queued queries: FP64 brute force over all references
+// 4. exact re-scan of queued queries: FP64 brute force over all references,
+//    one workgroup (4 waves) per queued query.
 // ------------------------------------------------------------------------
-template <bool LIMB>
-__device__ __forceinline__ int value_at(const int16_t* c16, const int8_t* lo, const int8_t* hi,
-                                        int64_t o) {
-    if (LIMB) return (int)lo[o] + (hi ? 256 * (int)hi[o] : 0);
-    return (int)c16[o];
-}
-
-template <bool LIMB>
+template <typename Rows>
 __global__ void __launch_bounds__(256)
-rescan_kernel(const int16_t* __restrict__ q16, const int8_t* __restrict__ qlo,
-              const int8_t* __restrict__ qhi, const int64_t* __restrict__ qn,
-              const int16_t* __restrict__ r16, const int8_t* __restrict__ rlo,
-              const int8_t* __restrict__ rhi, const int64_t* __restrict__ rn, int nr, int64_t ld,
-              int dim, int k, const int32_t* __restrict__ flag_list,
-              const int32_t* __restrict__ flag_count, int32_t* __restrict__ out_idx,
-              float* __restrict__ out_score, double* __restrict__ out_cos) {
-    __shared__ double s_c[256 * kMaxTopK / 8];
-    __shared__ int s_i[256 * kMaxTopK / 8];
-    const int tid = threadIdx.x;
+rescan_kernel(Rows rows, const int64_t* __restrict__ qn, const int64_t* __restrict__ rn, int nr,
+              int k, const int32_t* __restrict__ flag_list, const int32_t* __restrict__ flag_count,
+              int32_t* __restrict__ out_idx, float* __restrict__ out_score,
+              double* __restrict__ out_cos) {
+    __shared__ double s_c[4][kMaxTopK];
+    __shared__ int s_i[4][kMaxTopK];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int nflag = *flag_count;
     for (int f = blockIdx.x; f < nflag; f += gridDim.x) {
         const int m = flag_list[f];
-        double bc[kMaxTopK];
-        int bi[kMaxTopK];
-        int cnt = 0;
-        for (int n = tid; n < nr; n += 256) {
-            int64_t dot = 0;
-            for (int d = 0; d < dim; ++d)
-                dot += (int64_t)value_at<LIMB>(q16, qlo, qhi, (int64_t)m * ld + d) *
-                       (int64_t)value_at<LIMB>(r16, rlo, rhi, (int64_t)n * ld + d);
-            const double c = cos64(dot, qn[m], rn[n]);
-            if (cnt == k && (c < bc[k - 1] || (c == bc[k - 1] && n > bi[k - 1]))) continue;
-            int p = cnt < k ? cnt : k - 1;
-            while (p > 0 && (bc[p - 1] < c || (bc[p - 1] == c && bi[p - 1] > n))) {
-                bc[p] = bc[p - 1];
-                bi[p] = bi[p - 1];
-                --p;
-            }
-            bc[p] = c;
-            bi[p] = n;
-            if (cnt < k) ++cnt;
-        }
-        // merge the per-thread lists 8 threads at a time through LDS
-        const int per = kMaxTopK / 8 < k ? kMaxTopK / 8 : k;  // unused guard
-        (void)per;
-        for (int j = cnt; j < k; ++j) { bc[j] = -INFINITY; bi[j] = 0x7FFFFFFF; }
-        // sequential merge by thread 0 over all threads' lists, in rounds
-        __shared__ double m_c[kMaxTopK];
-        __shared__ int m_i[kMaxTopK];
-        if (tid == 0)
-            for (int j = 0; j < k; ++j) { m_c[j] = -INFINITY; m_i[j] = 0x7FFFFFFF; }
-        __syncthreads();
-        const int group = (256 * kMaxTopK / 8) / k;  // threads per round
-        for (int g0 = 0; g0 < 256; g0 += group) {
-            if (tid >= g0 && tid < g0 + group)
-                for (int j = 0; j < k; ++j) {
-                    s_c[(tid - g0) * k + j] = bc[j];
-                    s_i[(tid - g0) * k + j] = bi[j];
-                }
-            __syncthreads();
-            if (tid == 0) {
-                const int lim = (256 - g0 < group ? 256 - g0 : group) * k;
-                for (int e = 0; e < lim; ++e) {
-                    const double c = s_c[e];
-                    const int n = s_i[e];
-                    if (c < m_c[k - 1] || (c == m_c[k - 1] && n > m_i[k - 1])) continue;
+        const int64_t qnm = qn[m];
+        double* bc = s_c[wv];
+        int* bi = s_i[wv];
+        if (lane == 0)
+            for (int j = 0; j < k; ++j) { bc[j] = -INFINITY; bi[j] = 0x7FFFFFFF; }
+        for (int n = wv; n < nr; n += 4) {
+            const int64_t d = rows.dot(lane, m, n);
+            if (lane == 0) {
+                const double c = cos64(d, qnm, rn[n]);
+                if (c > bc[k - 1] || (c == bc[k - 1] && n < bi[k - 1])) {
                     int p = k - 1;
-                    while (p > 0 && (m_c[p - 1] < c || (m_c[p - 1] == c && m_i[p - 1] > n))) {
-                        m_c[p] = m_c[p - 1];
-                        m_i[p] = m_i[p - 1];
+                    while (p > 0 && (bc[p - 1] < c || (bc[p - 1] == c && bi[p - 1] > n))) {
+                        bc[p] = bc[p - 1];
+                        bi[p] = bi[p - 1];
                         --p;
                     }
-                    m_c[p] = c;
-                    m_i[p] = n;
+                    bc[p] = c;
+                    bi[p] = n;
                 }
             }
-            __syncthreads();
         }
-        if (tid < k) {
-            const int64_t o = (int64_t)m * k + tid;
-            const bool ok = m_i[tid] != 0x7FFFFFFF && m_i[tid] < nr;
-            out_idx[o] = ok ? m_i[tid] : -1;
-            out_score[o] = ok ? (float)m_c[tid] : -INFINITY;
-            if (out_cos) out_cos[o] = ok ? m_c[tid] : -INFINITY;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int w = 1; w < 4; ++w)  // merge the wave lists into wave 0's
+                for (int j = 0; j < k; ++j) {
+                    const double c = s_c[w][j];
+                    const int n = s_i[w][j];
+                    if (n == 0x7FFFFFFF) break;
+                    if (!(c > s_c[0][k - 1] || (c == s_c[0][k - 1] && n < s_i[0][k - 1]))) continue;
+                    int p = k - 1;
+                    while (p > 0 && (s_c[0][p - 1] < c || (s_c[0][p - 1] == c && s_i[0][p - 1] > n))) {
+                        s_c[0][p] = s_c[0][p - 1];
+                        s_i[0][p] = s_i[0][p - 1];
+                        --p;
+                    }
+                    s_c[0][p] = c;
+                    s_i[0][p] = n;
+                }
+            for (int j = 0; j < k; ++j) {
+                const int64_t o = (int64_t)m * k + j;
+                const bool ok = s_i[0][j] != 0x7FFFFFFF;
+                out_idx[o] = ok ? s_i[0][j] : -1;
+                out_score[o] = ok ? (float)s_c[0][j] : -INFINITY;
+                if (out_cos) out_cos[o] = ok ? s_c[0][j] : -INFINITY;
+            }
         }
         __syncthreads();
     }
@@ -563,34 +688,49 @@ int pick_kp(int k) { return k <= 5 ? 8 : (k <= 13 ? 16 : 32); }
 struct SearchBufs {
     float* iq;
     float* ir;
-    Cand* part;
+    uint64_t* part;
     int32_t* flag_list;
     int32_t* flag_count;
 };
 
 int prepare(hvk_ctx* ctx, int nq, int nr, int n_splits, int kp, SearchBufs& b) {
-    size_t need_n = sizeof(float) * ((size_t)nq + nr + 64);
+    const size_t need_n = sizeof(float) * ((size_t)nq + nr + 64);
     int st = ctx->ws_norms.ensure(need_n);
     if (st) return st;
-    st = ctx->ws_search.ensure(sizeof(Cand) * (size_t)n_splits * nq * kp);
+    st = ctx->ws_search.ensure(sizeof(uint64_t) * (size_t)n_splits * nq * kp);
     if (st) return st;
     st = ctx->ws_flags.ensure(sizeof(int32_t) * ((size_t)nq + 64));
     if (st) return st;
     b.iq = (float*)ctx->ws_norms.ptr;
     b.ir = b.iq + ((nq + 63) / 64) * 64;
-    b.part = (Cand*)ctx->ws_search.ptr;
+    b.part = (uint64_t*)ctx->ws_search.ptr;
     b.flag_count = (int32_t*)ctx->ws_flags.ptr;
     b.flag_list = b.flag_count + 64;
     return HVK_OK;
 }
 
+// splits of the reference range so that q_tiles * splits fills the device
 int choose_splits(hvk_ctx* ctx, int q_tiles, int r_tiles, int wg_per_cu) {
-    const int want = 2 * ctx->n_cu * wg_per_cu;
+    const int want = ctx->n_cu * wg_per_cu;
     int s = (want + q_tiles - 1) / q_tiles;
     if (s < 1) s = 1;
     if (s > r_tiles) s = r_tiles;
     if (s > 64) s = 64;
     return s;
+}
+
+template <typename Rows, int KP>
+int finish(hvk_ctx* ctx, const Rows& rows, const SearchBufs& b, int S, const int64_t* qn, int nq,
+           const int64_t* rn, int nr, int k, int32_t* idx, float* score, double* cosv,
+           int32_t* n_res, hipStream_t st) {
+    hipLaunchKernelGGL((finalize_kernel<KP, Rows>), dim3((nq + 3) / 4), dim3(256), 0, st, rows,
+                       b.part, S, nq, qn, rn, nr, k, idx, score, cosv, b.flag_list, b.flag_count);
+    hipLaunchKernelGGL((rescan_kernel<Rows>), dim3(ctx->n_cu), dim3(256), 0, st, rows, qn, rn, nr,
+                       k, b.flag_list, b.flag_count, idx, score, cosv);
+    HVK_HIP(hipGetLastError());
+    if (n_res)
+        HVK_HIP(hipMemcpyAsync(n_res, b.flag_count, sizeof(int32_t), hipMemcpyDeviceToDevice, st));
+    return HVK_OK;
 }
 
 template <int KP>
@@ -599,7 +739,7 @@ int run_plain(hvk_ctx* ctx, const int16_t* q, const int64_t* qn, int nq, const i
               double* cosv, int32_t* n_res, hipStream_t st) {
     const int q_tiles = (nq + kPlainMT - 1) / kPlainMT;
     const int r_tiles = (nr + kPlainNT - 1) / kPlainNT;
-    const int S = choose_splits(ctx, q_tiles, r_tiles, 2);
+    const int S = choose_splits(ctx, q_tiles, r_tiles, 4);
     const int split_tiles = (r_tiles + S - 1) / S;
     const int S_used = (r_tiles + split_tiles - 1) / split_tiles;
     SearchBufs b;
@@ -610,26 +750,20 @@ int run_plain(hvk_ctx* ctx, const int16_t* q, const int64_t* qn, int nq, const i
                        b.iq, b.ir, b.flag_count);
     timing_mark(ctx, 1, false, st);
     hipLaunchKernelGGL((plain_kernel<KP>), dim3(q_tiles, S_used), dim3(256), 0, st, q, b.iq, nq, r,
-                       b.ir, nr, ld, S_used, split_tiles, b.part);
+                       b.ir, nr, ld, split_tiles, b.part);
     timing_mark(ctx, 1, true, st);
-    hipLaunchKernelGGL((finalize_kernel<KP>), dim3((nq + 127) / 128), dim3(128), 0, st, b.part,
-                       S_used, nq, qn, rn, nr, k, idx, score, cosv, b.flag_list, b.flag_count);
-    hipLaunchKernelGGL((rescan_kernel<false>), dim3(ctx->n_cu), dim3(256), 0, st, q, nullptr,
-                       nullptr, qn, r, nullptr, nullptr, rn, nr, ld, ctx->dim, k, b.flag_list,
-                       b.flag_count, idx, score, cosv);
     HVK_HIP(hipGetLastError());
-    if (n_res)
-        HVK_HIP(hipMemcpyAsync(n_res, b.flag_count, sizeof(int32_t), hipMemcpyDeviceToDevice, st));
-    return HVK_OK;
+    return finish<RowsI16, KP>(ctx, RowsI16{q, r, ld}, b, S_used, qn, nq, rn, nr, k, idx, score,
+                               cosv, n_res, st);
 }
 
 template <int QL, int RL, int KP>
 int run_mfma(hvk_ctx* ctx, const int8_t* qlo, const int8_t* qhi, const int64_t* qn, int nq,
              const int8_t* rlo, const int8_t* rhi, const int64_t* rn, int nr, int64_t ld, int k,
              int32_t* idx, float* score, double* cosv, int32_t* n_res, hipStream_t st) {
-    const int q_tiles = (nq + kMfmaMT - 1) / kMfmaMT;
+    const int q_tiles = (nq + mfma_mt<QL>() - 1) / mfma_mt<QL>();
     const int r_tiles = (nr + kMfmaNT - 1) / kMfmaNT;
-    const int S = choose_splits(ctx, q_tiles, r_tiles, (QL * RL > 2) ? 1 : 2);
+    const int S = choose_splits(ctx, q_tiles, r_tiles, 1);
     const int split_tiles = (r_tiles + S - 1) / S;
     const int S_used = (r_tiles + split_tiles - 1) / split_tiles;
     SearchBufs b;
@@ -639,18 +773,13 @@ int run_mfma(hvk_ctx* ctx, const int8_t* qlo, const int8_t* qhi, const int64_t* 
     hipLaunchKernelGGL(inv_norm_kernel, dim3((nmax + 255) / 256), dim3(256), 0, st, qn, nq, rn, nr,
                        b.iq, b.ir, b.flag_count);
     timing_mark(ctx, 1, false, st);
-    hipLaunchKernelGGL((mfma_kernel<QL, RL, KP>), dim3(q_tiles, S_used), dim3(256), 0, st, qlo, qhi,
-                       b.iq, nq, rlo, rhi, b.ir, nr, ld, S_used, split_tiles, b.part);
+    hipLaunchKernelGGL((mfma_kernel<QL, RL, KP>), dim3(q_tiles, S_used), dim3(kMfmaThreads), 0, st,
+                       qlo, qhi, b.iq, nq, rlo, rhi, b.ir, nr, ld, split_tiles, b.part);
     timing_mark(ctx, 1, true, st);
-    hipLaunchKernelGGL((finalize_kernel<KP>), dim3((nq + 127) / 128), dim3(128), 0, st, b.part,
-                       S_used, nq, qn, rn, nr, k, idx, score, cosv, b.flag_list, b.flag_count);
-    hipLaunchKernelGGL((rescan_kernel<true>), dim3(ctx->n_cu), dim3(256), 0, st, nullptr, qlo, qhi,
-                       qn, nullptr, rlo, rhi, rn, nr, ld, ctx->dim, k, b.flag_list, b.flag_count,
-                       idx, score, cosv);
     HVK_HIP(hipGetLastError());
-    if (n_res)
-        HVK_HIP(hipMemcpyAsync(n_res, b.flag_count, sizeof(int32_t), hipMemcpyDeviceToDevice, st));
-    return HVK_OK;
+    return finish<RowsLimb, KP>(ctx, RowsLimb{qlo, QL > 1 ? qhi : nullptr, rlo,
+                                              RL > 1 ? rhi : nullptr, ld},
+                                b, S_used, qn, nq, rn, nr, k, idx, score, cosv, n_res, st);
 }
 
 template <int QL, int RL>
@@ -682,8 +811,7 @@ extern "C" int hvk_search_plain(hvk_ctx* ctx, const int16_t* d_q, const int64_t*
     if (n_r == 0) return HVK_ERR_ARG;
     std::lock_guard<std::mutex> g(ctx->mu);
     hipStream_t st = (hipStream_t)stream;
-    const int kp = hvk::pick_kp(k);
-    switch (kp) {
+    switch (hvk::pick_kp(k)) {
         case 8: return hvk::run_plain<8>(ctx, d_q, d_q_sqnorm, n_q, d_r, d_r_sqnorm, n_r, ld, k, d_idx, d_score, d_cos64, d_n_rescored, st);
         case 16: return hvk::run_plain<16>(ctx, d_q, d_q_sqnorm, n_q, d_r, d_r_sqnorm, n_r, ld, k, d_idx, d_score, d_cos64, d_n_rescored, st);
         default: return hvk::run_plain<32>(ctx, d_q, d_q_sqnorm, n_q, d_r, d_r_sqnorm, n_r, ld, k, d_idx, d_score, d_cos64, d_n_rescored, st);

@@ -85,13 +85,23 @@ def test_exact_ties_duplicate_references(workload, form):
     the near-tie margin check must route the queries to the exact re-scan."""
     import hdc_amd as H
     ctx, refs, reads, rc, qc, w = workload
+    # pairs of duplicates: ties inside the candidate list, no re-scan needed
     dup = np.concatenate([rc[:40], rc[:40], rc[5:6], rc[5:6], rc[5:6], rc[40:90]])
     db = encoded_from_counts(dup, ctx.ld)
     q = encoded_from_counts(qc[:33], ctx.ld)
     ref_idx, ref_cos = orc.topk16(qc[:33], dup, 10)
     res = (H.search_plain if form == "plain" else H.search_mfma)(ctx, q, db, 10)
     _check(res, ref_idx, ref_cos, 10)
-    assert int(res.n_rescored.cpu().item()) > 0
+    # 21 copies tie at c = 1 for queries equal to reference 5: the tie group
+    # outgrows the FP32 candidate list (16), so those queries are re-scanned
+    big = np.concatenate([rc[:30], np.repeat(rc[5:6], 20, axis=0), rc[30:60]])
+    qn = np.concatenate([rc[5:6], qc[:7], rc[5:6]])
+    ref_idx, ref_cos = orc.topk16(qn, big, 10)
+    res = (H.search_plain if form == "plain" else H.search_mfma)(
+        ctx, encoded_from_counts(qn, ctx.ld), encoded_from_counts(big, ctx.ld), 10)
+    _check(res, ref_idx, ref_cos, 10)
+    assert list(ref_idx[0]) == [5] + list(range(30, 39))
+    assert int(res.n_rescored.cpu().item()) >= 2
 
 
 @pytest.mark.parametrize("form", ["plain", "mfma"])
