@@ -82,6 +82,12 @@ __device__ __forceinline__ int64_t wave_sum64(int64_t v) {
     return v;
 }
 
+__device__ __forceinline__ int wave_sum(int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
 __device__ __forceinline__ int wave_max(int v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
@@ -92,16 +98,15 @@ template <int K, int P>
 __global__ void __launch_bounds__(1024)
 encode_kernel(const uint32_t* __restrict__ gtable, int k_rt, int D, int W32,
               const uint32_t* __restrict__ packed, const uint32_t* __restrict__ invalid,
-              const int64_t* __restrict__ offsets, int n_seqs, int16_t* __restrict__ counts,
-              int8_t* __restrict__ lo, int8_t* __restrict__ hi, int64_t ld,
-              int64_t* __restrict__ sqnorm, int32_t* __restrict__ maxabs,
-              int32_t* __restrict__ nkmers) {
+              const int64_t* __restrict__ offsets, const int64_t* __restrict__ starts,
+              int window_len, int n_seqs, EncodeOut out) {
     const int k = K > 0 ? K : k_rt;
     extern __shared__ uint32_t smem[];
     uint32_t* tab = smem;
     uint32_t* codes = smem + 4 * k * W32;
     __shared__ int64_t red_sq[16];
     __shared__ int red_mx[16];
+    __shared__ int red_res[16];
 
     const int tid = threadIdx.x;
     const int nthr = blockDim.x;
@@ -114,8 +119,9 @@ encode_kernel(const uint32_t* __restrict__ gtable, int k_rt, int D, int W32,
     const int kW = k * W32;
 
     for (int s = blockIdx.x; s < n_seqs; s += gridDim.x) {
-        const int64_t off = offsets[s];
-        const int64_t len = offsets[s + 1] - off;
+        // sequence s: [offsets[s], offsets[s+1]) or the window starting at starts[s]
+        const int64_t off = starts ? starts[s] : offsets[s];
+        const int64_t len = starts ? (int64_t)window_len : offsets[s + 1] - off;
         const int64_t npos = len - k + 1;
         uint32_t planes[P];
 #pragma unroll
@@ -170,12 +176,13 @@ encode_kernel(const uint32_t* __restrict__ gtable, int k_rt, int D, int W32,
 
         // ---- phase 3: per-dimension counts ----
         int64_t sq = 0;
-        int mx = 0;
+        int mx = 0, res = 0;
+        const int64_t ld = out.ld;
         if (w < W32) {
             uint32_t packed16[16];
-            uint32_t plo[8], phi[8];
+            uint32_t plo[8], phi[8], pcl[8];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) { plo[q] = 0; phi[q] = 0; }
+            for (int q = 0; q < 8; ++q) { plo[q] = 0; phi[q] = 0; pcl[q] = 0; }
 #pragma unroll
             for (int b = 0; b < 32; ++b) {
                 uint32_t pc = 0;
@@ -183,52 +190,68 @@ encode_kernel(const uint32_t* __restrict__ gtable, int k_rt, int D, int W32,
                 for (int p = 0; p < P; ++p) pc |= ((planes[p] >> b) & 1u) << p;
                 int cnt = (32 * w + b < D) ? (int)(int16_t)(nvalid - 2 * (int)pc) : 0;
                 sq += (int64_t)cnt * cnt;
-                mx = max(mx, cnt < 0 ? -cnt : cnt);
+                const int a = cnt < 0 ? -cnt : cnt;
+                mx = max(mx, a);
+                res += a > 127 ? a - 127 : 0;
                 const uint32_t u16 = (uint32_t)(uint16_t)(int16_t)cnt;
                 if (b & 1) packed16[b >> 1] |= u16 << 16;
                 else packed16[b >> 1] = u16;
                 const int l8 = (int)(int8_t)(cnt & 0xFF);
                 const int h8 = (cnt - l8) >> 8;
+                const int c8 = cnt > 127 ? 127 : (cnt < -127 ? -127 : cnt);
                 plo[b >> 2] |= ((uint32_t)(uint8_t)l8) << (8 * (b & 3));
                 phi[b >> 2] |= ((uint32_t)(uint8_t)h8) << (8 * (b & 3));
+                pcl[b >> 2] |= ((uint32_t)(uint8_t)c8) << (8 * (b & 3));
             }
             const int64_t row = (int64_t)s * ld + 32 * (int64_t)w;
-            if (counts) {
-                uint4* dst = reinterpret_cast<uint4*>(counts + row);
+            if (out.counts) {
+                uint4* dst = reinterpret_cast<uint4*>(out.counts + row);
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
                     dst[q] = make_uint4(packed16[4 * q], packed16[4 * q + 1], packed16[4 * q + 2],
                                         packed16[4 * q + 3]);
             }
-            if (lo) {
-                uint4* dst = reinterpret_cast<uint4*>(lo + row);
+            if (out.lo) {
+                uint4* dst = reinterpret_cast<uint4*>(out.lo + row);
                 dst[0] = make_uint4(plo[0], plo[1], plo[2], plo[3]);
                 dst[1] = make_uint4(plo[4], plo[5], plo[6], plo[7]);
             }
-            if (hi) {
-                uint4* dst = reinterpret_cast<uint4*>(hi + row);
+            if (out.hi) {
+                uint4* dst = reinterpret_cast<uint4*>(out.hi + row);
                 dst[0] = make_uint4(phi[0], phi[1], phi[2], phi[3]);
                 dst[1] = make_uint4(phi[4], phi[5], phi[6], phi[7]);
+            }
+            if (out.clamp8) {
+                uint4* dst = reinterpret_cast<uint4*>(out.clamp8 + row);
+                dst[0] = make_uint4(pcl[0], pcl[1], pcl[2], pcl[3]);
+                dst[1] = make_uint4(pcl[4], pcl[5], pcl[6], pcl[7]);
             }
         }
         // zero padding columns beyond the last full word
         for (int64_t d = 32 * (int64_t)W32 + tid; d < ld; d += nthr) {
             const int64_t o = (int64_t)s * ld + d;
-            if (counts) counts[o] = 0;
-            if (lo) lo[o] = 0;
-            if (hi) hi[o] = 0;
+            if (out.counts) out.counts[o] = 0;
+            if (out.lo) out.lo[o] = 0;
+            if (out.hi) out.hi[o] = 0;
+            if (out.clamp8) out.clamp8[o] = 0;
         }
         sq = wave_sum64(sq);
         mx = wave_max(mx);
-        if (lane == 0) { red_sq[wave] = sq; red_mx[wave] = mx; }
+        res = wave_sum(res);
+        if (lane == 0) { red_sq[wave] = sq; red_mx[wave] = mx; red_res[wave] = res; }
         __syncthreads();
         if (tid == 0) {
             int64_t t = 0;
-            int m = 0;
-            for (int q = 0; q < nwaves; ++q) { t += red_sq[q]; m = max(m, red_mx[q]); }
-            if (sqnorm) sqnorm[s] = t;
-            if (maxabs) maxabs[s] = m;
-            if (nkmers) nkmers[s] = nvalid;
+            int m = 0, r = 0;
+            for (int q = 0; q < nwaves; ++q) {
+                t += red_sq[q];
+                m = max(m, red_mx[q]);
+                r += red_res[q];
+            }
+            if (out.sqnorm) out.sqnorm[s] = t;
+            if (out.maxabs) out.maxabs[s] = m;
+            if (out.nkmers) out.nkmers[s] = nvalid;
+            if (out.resid1) out.resid1[s] = r;
         }
         __syncthreads();
     }
@@ -236,8 +259,8 @@ encode_kernel(const uint32_t* __restrict__ gtable, int k_rt, int D, int W32,
 
 template <int K, int P>
 int launch_kp(hvk_ctx* ctx, const uint32_t* packed, const uint32_t* invalid,
-              const int64_t* offsets, int n, int16_t* counts, int8_t* lo, int8_t* hi,
-              int64_t ld, int64_t* sqnorm, int32_t* maxabs, int32_t* nkmers, hipStream_t st) {
+              const int64_t* offsets, const int64_t* starts, int window_len, int n,
+              const EncodeOut& out, hipStream_t st) {
     const int W32 = ctx->w32;
     const int threads = (W32 + 63) / 64 * 64;
     const size_t lds = sizeof(uint32_t) * ((size_t)4 * ctx->k * W32 + kChunk);
@@ -245,8 +268,8 @@ int launch_kp(hvk_ctx* ctx, const uint32_t* packed, const uint32_t* invalid,
     int grid = n < 4 * ctx->n_cu ? n : 4 * ctx->n_cu;
     timing_mark(ctx, 0, false, st);
     hipLaunchKernelGGL((encode_kernel<K, P>), dim3(grid), dim3(threads), lds, st, ctx->d_table,
-                       ctx->k, ctx->dim, W32, packed, invalid, offsets, n, counts, lo, hi, ld,
-                       sqnorm, maxabs, nkmers);
+                       ctx->k, ctx->dim, W32, packed, invalid, offsets, starts, window_len, n,
+                       out);
     timing_mark(ctx, 0, true, st);
     HVK_HIP(hipGetLastError());
     return HVK_OK;
@@ -254,43 +277,91 @@ int launch_kp(hvk_ctx* ctx, const uint32_t* packed, const uint32_t* invalid,
 
 template <int K>
 int launch_k(hvk_ctx* ctx, int max_len, const uint32_t* packed, const uint32_t* invalid,
-             const int64_t* offsets, int n, int16_t* counts, int8_t* lo, int8_t* hi, int64_t ld,
-             int64_t* sqnorm, int32_t* maxabs, int32_t* nkmers, hipStream_t st) {
+             const int64_t* offsets, const int64_t* starts, int window_len, int n,
+             const EncodeOut& out, hipStream_t st) {
     const int64_t nmax = (int64_t)max_len - ctx->k + 1;
     if (nmax < (1 << 11))
-        return launch_kp<K, 11>(ctx, packed, invalid, offsets, n, counts, lo, hi, ld, sqnorm, maxabs, nkmers, st);
+        return launch_kp<K, 11>(ctx, packed, invalid, offsets, starts, window_len, n, out, st);
     if (nmax < (1 << 16))
-        return launch_kp<K, 16>(ctx, packed, invalid, offsets, n, counts, lo, hi, ld, sqnorm, maxabs, nkmers, st);
-    return launch_kp<K, 24>(ctx, packed, invalid, offsets, n, counts, lo, hi, ld, sqnorm, maxabs, nkmers, st);
+        return launch_kp<K, 16>(ctx, packed, invalid, offsets, starts, window_len, n, out, st);
+    return launch_kp<K, 24>(ctx, packed, invalid, offsets, starts, window_len, n, out, st);
 }
 
 }  // namespace
 
 int launch_encode(hvk_ctx* ctx, const uint32_t* packed, const uint32_t* invalid,
-                  const int64_t* offsets, int n, int max_len, int16_t* counts, int8_t* lo,
-                  int8_t* hi, int64_t ld, int64_t* sqnorm, int32_t* maxabs, int32_t* nkmers,
-                  hipStream_t st) {
+                  const int64_t* offsets, const int64_t* starts, int window_len, int n,
+                  int max_len, const EncodeOut& out, hipStream_t st) {
     switch (ctx->k) {
-        case 5: return launch_k<5>(ctx, max_len, packed, invalid, offsets, n, counts, lo, hi, ld, sqnorm, maxabs, nkmers, st);
-        case 10: return launch_k<10>(ctx, max_len, packed, invalid, offsets, n, counts, lo, hi, ld, sqnorm, maxabs, nkmers, st);
-        case 12: return launch_k<12>(ctx, max_len, packed, invalid, offsets, n, counts, lo, hi, ld, sqnorm, maxabs, nkmers, st);
-        default: return launch_k<0>(ctx, max_len, packed, invalid, offsets, n, counts, lo, hi, ld, sqnorm, maxabs, nkmers, st);
+        case 5: return launch_k<5>(ctx, max_len, packed, invalid, offsets, starts, window_len, n, out, st);
+        case 10: return launch_k<10>(ctx, max_len, packed, invalid, offsets, starts, window_len, n, out, st);
+        case 12: return launch_k<12>(ctx, max_len, packed, invalid, offsets, starts, window_len, n, out, st);
+        default: return launch_k<0>(ctx, max_len, packed, invalid, offsets, starts, window_len, n, out, st);
     }
 }
 
 }  // namespace hvk
 
+namespace {
+int check_out(const hvk_ctx* ctx, hvk::EncodeOut& o) {
+    if (o.ld == 0) o.ld = ctx->ld;
+    if (o.ld < 32 * (int64_t)ctx->w32 || (o.ld % 16) != 0) return HVK_ERR_ARG;
+    return HVK_OK;
+}
+hvk::EncodeOut to_out(const hvk_encode_out* o) {
+    hvk::EncodeOut e{};
+    if (!o) return e;
+    e.counts = o->counts;
+    e.lo = o->lo;
+    e.hi = o->hi;
+    e.clamp8 = o->clamp8;
+    e.ld = o->ld;
+    e.sqnorm = o->sqnorm;
+    e.maxabs = o->maxabs;
+    e.nkmers = o->nkmers;
+    e.resid1 = o->resid1;
+    return e;
+}
+}  // namespace
+
+extern "C" int hvk_encode_ex(hvk_ctx* ctx, const uint32_t* d_packed, const uint32_t* d_invalid,
+                             const int64_t* d_offsets, int n_seqs, int max_len,
+                             const hvk_encode_out* out, hvk_stream stream) {
+    if (!ctx || !d_packed || !d_invalid || !d_offsets || !out || n_seqs < 0 || max_len < 0)
+        return HVK_ERR_ARG;
+    if (n_seqs == 0) return HVK_OK;
+    hvk::EncodeOut o = to_out(out);
+    if (int st = check_out(ctx, o)) return st;
+    if ((int64_t)max_len - ctx->k + 1 >= (1 << 24)) return HVK_ERR_RANGE;
+    return hvk::launch_encode(ctx, d_packed, d_invalid, d_offsets, nullptr, 0, n_seqs, max_len, o,
+                              (hipStream_t)stream);
+}
+
+extern "C" int hvk_encode_windows(hvk_ctx* ctx, const uint32_t* d_packed,
+                                  const uint32_t* d_invalid, const int64_t* d_starts,
+                                  int n_windows, int window_len, const hvk_encode_out* out,
+                                  hvk_stream stream) {
+    if (!ctx || !d_packed || !d_invalid || !d_starts || !out || n_windows < 0 || window_len < 0)
+        return HVK_ERR_ARG;
+    if (n_windows == 0) return HVK_OK;
+    hvk::EncodeOut o = to_out(out);
+    if (int st = check_out(ctx, o)) return st;
+    if ((int64_t)window_len - ctx->k + 1 >= (1 << 24)) return HVK_ERR_RANGE;
+    return hvk::launch_encode(ctx, d_packed, d_invalid, nullptr, d_starts, window_len, n_windows,
+                              window_len, o, (hipStream_t)stream);
+}
+
 extern "C" int hvk_encode(hvk_ctx* ctx, const uint32_t* d_packed, const uint32_t* d_invalid,
                           const int64_t* d_offsets, int n_seqs, int max_len, int16_t* d_counts,
                           int8_t* d_lo, int8_t* d_hi, int64_t ld, int64_t* d_sqnorm,
                           int32_t* d_maxabs, int32_t* d_nkmers, hvk_stream stream) {
-    if (!ctx || !d_packed || !d_invalid || !d_offsets || n_seqs < 0 || max_len < 0)
-        return HVK_ERR_ARG;
-    if (n_seqs == 0) return HVK_OK;
-    if (ld == 0) ld = ctx->ld;
-    if (ld < 32 * (int64_t)ctx->w32 || (ld % 16) != 0) return HVK_ERR_ARG;
-    if ((int64_t)max_len - ctx->k + 1 >= (1 << 24)) return HVK_ERR_RANGE;
-    return hvk::launch_encode(ctx, d_packed, d_invalid, d_offsets, n_seqs, max_len, d_counts,
-                              d_lo, d_hi, ld, d_sqnorm, d_maxabs, d_nkmers,
-                              (hipStream_t)stream);
+    hvk_encode_out o{};
+    o.counts = d_counts;
+    o.lo = d_lo;
+    o.hi = d_hi;
+    o.ld = ld;
+    o.sqnorm = d_sqnorm;
+    o.maxabs = d_maxabs;
+    o.nkmers = d_nkmers;
+    return hvk_encode_ex(ctx, d_packed, d_invalid, d_offsets, n_seqs, max_len, &o, stream);
 }

@@ -54,7 +54,8 @@ struct hvk_ctx {
     uint32_t* d_table = nullptr;        // [4][k][w32] rotated base words
     hvk::Workspace ws_search;           // partial top lists
     hvk::Workspace ws_norms;            // inverse norms
-    hvk::Workspace ws_flags;            // rescan queue
+    hvk::Workspace ws_flags;            // rescan queues
+    hvk::Workspace ws_search2;          // partial lists of the exact second pass
     std::mutex mu;
     // kernel timing (hvk_ctx_set_timing)
     bool timing = false;
@@ -70,8 +71,21 @@ inline void timing_mark(hvk_ctx* c, int which, bool stop, hipStream_t s) {
     (void)hipEventRecord(c->ev[2 * which + (stop ? 1 : 0)], s);
     if (stop) c->ev_used[which] = true;
 }
+// Outputs of the encode kernel (each pointer nullable; ld = row stride).
+struct EncodeOut {
+    int16_t* counts;
+    int8_t* lo;
+    int8_t* hi;
+    int8_t* clamp8;
+    int64_t ld;
+    int64_t* sqnorm;
+    int32_t* maxabs;
+    int32_t* nkmers;
+    int32_t* resid1;
+};
+// Sequences are [offsets[i], offsets[i+1]) or, when starts is set, windows of
+// window_len bases starting at starts[i].
 int launch_encode(hvk_ctx* ctx, const uint32_t* packed, const uint32_t* invalid,
-                  const int64_t* offsets, int n, int max_len, int16_t* counts,
-                  int8_t* lo, int8_t* hi, int64_t ld, int64_t* sqnorm, int32_t* maxabs,
-                  int32_t* nkmers, hipStream_t s);
+                  const int64_t* offsets, const int64_t* starts, int window_len, int n,
+                  int max_len, const EncodeOut& out, hipStream_t s);
 }

@@ -129,18 +129,37 @@ __device__ __forceinline__ bool topk_pass_end(TopLds<MT, KP, CAP>& tl, int tid, 
 // ------------------------------------------------------------------------
 // 1. inverse norms; also clears the re-scan queue counter
 // ------------------------------------------------------------------------
+struct BoundRows {  // nullable inputs of the bounded form
+    const int32_t* q_maxabs;
+    const int32_t* q_resid1;
+    const int32_t* r_maxabs;
+    const int32_t* r_resid1;
+};
+
+// bq[m] = (min(max|q|,127) iq, resid1(q) iq), br[n] = (resid1(r) ir, max|r| ir):
+// sum_j bq[m].j * br[n].j bounds |dot - dot_clamped| * iq * ir.
 __global__ void inv_norm_kernel(const int64_t* __restrict__ qn, int nq, const int64_t* __restrict__ rn,
                                 int nr, float* __restrict__ iq, float* __restrict__ ir,
-                                int32_t* __restrict__ flag_count) {
+                                int32_t* __restrict__ flag_count, BoundRows bnd,
+                                float2* __restrict__ bq, float2* __restrict__ br) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) *flag_count = 0;
+    if (i == 0) { flag_count[0] = 0; flag_count[32] = 0; }
     if (i < nq) {
         const int64_t v = qn[i];
-        iq[i] = v == 0 ? 0.0f : __fdiv_rn(1.0f, __fsqrt_rn((float)v));
+        const float x = v == 0 ? 0.0f : __fdiv_rn(1.0f, __fsqrt_rn((float)v));
+        iq[i] = x;
+        if (bq) {
+            const int a = bnd.q_maxabs[i] < 127 ? bnd.q_maxabs[i] : 127;
+            bq[i] = make_float2(__fmul_rn((float)a, x), __fmul_rn((float)bnd.q_resid1[i], x));
+        }
     }
     if (i < nr) {
         const int64_t v = rn[i];
-        ir[i] = v == 0 ? 0.0f : __fdiv_rn(1.0f, __fsqrt_rn((float)v));
+        const float x = v == 0 ? 0.0f : __fdiv_rn(1.0f, __fsqrt_rn((float)v));
+        ir[i] = x;
+        if (br)
+            br[i] = make_float2(__fmul_rn((float)bnd.r_resid1[i], x),
+                                __fmul_rn((float)bnd.r_maxabs[i], x));
     }
 }
 
@@ -293,14 +312,22 @@ struct MfmaSmem {
     int8_t r[2][RL][kMfmaNT * kMfmaKS];
     TopLds<mfma_mt<QL>(), KP, 8> tl;
     float iq[mfma_mt<QL>()];
+    float2 bq[mfma_mt<QL>()];
 };
 
-template <int QL, int RL, int KP>
+// BND: one-limb bounded form -- both operands hold clamped counts and each
+// score is raised by bq[m].x br[n].x + bq[m].y br[n].y, an upper bound on
+// |dot - dot_clamped| * iq * ir.
+template <int QL, int RL, int KP, bool BND = false>
 __global__ void __launch_bounds__(kMfmaThreads, 2)
 mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
             const float* __restrict__ iq, int nq, const int8_t* __restrict__ rlo,
             const int8_t* __restrict__ rhi, const float* __restrict__ ir, int nr, int64_t ld,
-            int split_tiles, uint64_t* __restrict__ part) {
+            int split_tiles, uint64_t* __restrict__ part, const float2* __restrict__ bq,
+            const float2* __restrict__ br, const int32_t* __restrict__ q_rows,
+            const int32_t* __restrict__ q_count) {
+    // q_rows (nullable): query m of this launch is stored row q_rows[m], and
+    // the launch covers m < *q_count (a count on the device, at most nq).
     constexpr int WX = mfma_wx<QL>(), MT = mfma_mt<QL>(), NT = kMfmaNT, KS = kMfmaKS;
     constexpr int NTH = kMfmaThreads;
     constexpr int NPAIR = WX * 2 * 16;       // (query, reference) pairs per lane
@@ -312,8 +339,18 @@ mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
     const int wm = wave & 1, wn = wave >> 1;
     const int m0 = blockIdx.x * MT;
     const int split = blockIdx.y;
+    if (q_rows) {
+        const int c = *q_count;
+        nq = c < nq ? c : nq;
+        if (m0 >= nq) return;  // the whole workgroup leaves before any barrier
+    }
+    auto qrow = [&](int m) { return q_rows ? q_rows[m] : m; };
     tl.init(tid, NTH);
-    if (tid < MT) sm.iq[tid] = (m0 + tid < nq) ? iq[m0 + tid] : 0.0f;
+    if (tid < MT) {
+        const bool in = m0 + tid < nq;
+        sm.iq[tid] = in ? iq[qrow(m0 + tid)] : 0.0f;
+        sm.bq[tid] = (BND && in) ? bq[qrow(m0 + tid)] : make_float2(0.0f, 0.0f);
+    }
     const int n_begin = split * split_tiles * NT;
     int n_end = n_begin + split_tiles * NT;
     if (n_end > nr) n_end = nr;
@@ -351,7 +388,7 @@ mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
         for (int l = 0; l < QL; ++l)
 #pragma unroll
             for (int t = 0; t < QJ; ++t) {
-                const int row = min(m0 + q_rowi[t], nq - 1);
+                const int row = qrow(min(m0 + q_rowi[t], nq - 1));
                 gq[l][t] = qsrc[l] + (int64_t)row * ld + q_off[t];
             }
 #pragma unroll
@@ -414,10 +451,12 @@ mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
         }
 
         float irv[2];
+        float2 brv[2];
 #pragma unroll
         for (int y = 0; y < 2; ++y) {
             const int n = n0 + wn * 64 + y * 32 + (lane & 31);
             irv[y] = n < nr ? ir[n] : 0.0f;
+            brv[y] = (BND && n < nr) ? br[n] : make_float2(0.0f, 0.0f);
         }
         // Epilogue.  Pair p = (x * 2 + y) * 16 + e is query row
         // wm*32*WX + x*32 + 8(e/4) + 4(lane/32) + e%4, reference column
@@ -440,6 +479,13 @@ mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
                     const float4 th4 = *reinterpret_cast<const float4*>(&tl.thr_f[mb + 8 * g]);
                     const float iqv[4] = {iq4.x, iq4.y, iq4.z, iq4.w};
                     const float thv[4] = {th4.x, th4.y, th4.z, th4.w};
+                    float2 bqv[4];
+                    if constexpr (BND) {
+                        const float4 b01 = *reinterpret_cast<const float4*>(&sm.bq[mb + 8 * g]);
+                        const float4 b23 = *reinterpret_cast<const float4*>(&sm.bq[mb + 8 * g + 2]);
+                        bqv[0] = make_float2(b01.x, b01.y); bqv[1] = make_float2(b01.z, b01.w);
+                        bqv[2] = make_float2(b23.x, b23.y); bqv[3] = make_float2(b23.z, b23.w);
+                    }
                     static_for<4>([&](auto H) {
                         constexpr int h = decltype(H)::value, e = 4 * g + h;
                         constexpr int p = (x * 2 + y) * 16 + e;
@@ -450,6 +496,8 @@ mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
                             v = __fmaf_rn((float)acc[qa][rb][x][y][e], scale, v);
                         });
                         sv[e] = __fmul_rn(__fmul_rn(v, iqv[h]), irv[y]);
+                        if constexpr (BND)
+                            sv[e] = __fmaf_rn(bqv[h].y, brv[y].y, __fmaf_rn(bqv[h].x, brv[y].x, sv[e]));
                         const bool want = pass == 0 || ((dropped[p / 64] >> (p % 64)) & 1);
                         hit |= want && sv[e] >= thv[h];
                     });
@@ -550,19 +598,31 @@ finalize_kernel(Rows rows, const uint64_t* __restrict__ part, int n_splits, int 
                 const int64_t* __restrict__ qn, const int64_t* __restrict__ rn, int nr, int k,
                 int32_t* __restrict__ out_idx, float* __restrict__ out_score,
                 double* __restrict__ out_cos, int32_t* __restrict__ flag_list,
-                int32_t* __restrict__ flag_count) {
+                int32_t* __restrict__ flag_count, const int32_t* __restrict__ q_rows,
+                const int32_t* __restrict__ q_count, int part_rows) {
+    // q_rows (nullable): query m is stored row q_rows[m], m < *q_count; the
+    // partial lists hold part_rows queries, later ones are forwarded to the
+    // flag list unexamined.
     __shared__ uint64_t s_key[4][KP];
     __shared__ double s_c[4][KP];
     __shared__ int s_i[4][KP];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int m = blockIdx.x * 4 + wv;
-    if (m >= nq) return;  // whole wave exits together
+    int n_all = nq;
+    if (q_rows) n_all = *q_count < nq ? *q_count : nq;
+    if (m >= n_all) return;  // whole wave exits together
+    const int row = q_rows ? q_rows[m] : m;
+    if (m >= part_rows) {
+        if (lane == 0) flag_list[atomicAdd(flag_count, 1)] = row;
+        return;
+    }
+    const int np = n_all < part_rows ? n_all : part_rows;  // queries with partial lists
     uint64_t* key = s_key[wv];
     if (lane == 0) {
         for (int j = 0; j < KP; ++j) key[j] = part[(int64_t)m * KP + j];
         for (int s = 1; s < n_splits; ++s)
             for (int j = 0; j < KP; ++j) {
-                const uint64_t c = part[((int64_t)s * nq + m) * KP + j];
+                const uint64_t c = part[((int64_t)s * np + m) * KP + j];
                 if (c <= key[KP - 1]) break;  // each list is sorted
                 int p = KP - 1;
                 while (p > 0 && key[p - 1] < c) { key[p] = key[p - 1]; --p; }
@@ -572,10 +632,10 @@ finalize_kernel(Rows rows, const uint64_t* __restrict__ part, int n_splits, int 
     __builtin_amdgcn_wave_barrier();
     int nv = 0;
     while (nv < KP && key[nv] != 0) ++nv;
-    const int64_t qnm = qn[m];
+    const int64_t qnm = qn[row];
     for (int j = 0; j < nv; ++j) {
         const int idx = idx_of_key(key[j]);
-        const int64_t d = rows.dot(lane, m, idx);
+        const int64_t d = rows.dot(lane, row, idx);
         if (lane == 0) { s_i[wv][j] = idx; s_c[wv][j] = cos64(d, qnm, rn[idx]); }
     }
     __builtin_amdgcn_wave_barrier();
@@ -600,7 +660,7 @@ finalize_kernel(Rows rows, const uint64_t* __restrict__ part, int n_splits, int 
         flag = !(cs[k - 1] > (double)s_kp + (double)kMarginEps);
     }
     for (int j = 0; j < k; ++j) {
-        const int64_t o = (int64_t)m * k + j;
+        const int64_t o = (int64_t)row * k + j;
         if (j < nv) {
             out_idx[o] = ix[j];
             out_score[o] = (float)cs[j];
@@ -611,7 +671,7 @@ finalize_kernel(Rows rows, const uint64_t* __restrict__ part, int n_splits, int 
             if (out_cos) out_cos[o] = -INFINITY;
         }
     }
-    if (flag) flag_list[atomicAdd(flag_count, 1)] = m;
+    if (flag) flag_list[atomicAdd(flag_count, 1)] = row;
 }
 
 // ------------------------------------------------------------------------
@@ -688,24 +748,40 @@ int pick_kp(int k) { return k <= 5 ? 8 : (k <= 13 ? 16 : 32); }
 struct SearchBufs {
     float* iq;
     float* ir;
+    float2* bq;
+    float2* br;
     uint64_t* part;
-    int32_t* flag_list;
+    int32_t* flag_list;   // queued by the first finalize
     int32_t* flag_count;
+    uint64_t* part2;      // second (exact) pass
+    int32_t* flag_list2;  // queued by the second finalize
+    int32_t* flag_count2;
 };
 
+constexpr int kPass2Cap = 16384;  // queries the exact second pass takes at once
+constexpr int kPass2Splits = 16;
+constexpr int kPass2KP = 32;
+
 int prepare(hvk_ctx* ctx, int nq, int nr, int n_splits, int kp, SearchBufs& b) {
-    const size_t need_n = sizeof(float) * ((size_t)nq + nr + 64);
+    const size_t need_n = sizeof(float) * (3 * ((size_t)nq + nr) + 512);
     int st = ctx->ws_norms.ensure(need_n);
     if (st) return st;
     st = ctx->ws_search.ensure(sizeof(uint64_t) * (size_t)n_splits * nq * kp);
     if (st) return st;
-    st = ctx->ws_flags.ensure(sizeof(int32_t) * ((size_t)nq + 64));
+    st = ctx->ws_flags.ensure(sizeof(int32_t) * (2 * (size_t)nq + 128));
+    if (st) return st;
+    st = ctx->ws_search2.ensure(sizeof(uint64_t) * (size_t)kPass2Splits * kPass2Cap * kPass2KP);
     if (st) return st;
     b.iq = (float*)ctx->ws_norms.ptr;
     b.ir = b.iq + ((nq + 63) / 64) * 64;
+    b.bq = reinterpret_cast<float2*>(b.ir + ((nr + 63) / 64) * 64);
+    b.br = b.bq + ((nq + 63) / 64) * 64;
     b.part = (uint64_t*)ctx->ws_search.ptr;
     b.flag_count = (int32_t*)ctx->ws_flags.ptr;
+    b.flag_count2 = b.flag_count + 32;
     b.flag_list = b.flag_count + 64;
+    b.flag_list2 = b.flag_list + nq + 64;
+    b.part2 = (uint64_t*)ctx->ws_search2.ptr;
     return HVK_OK;
 }
 
@@ -720,17 +796,38 @@ int choose_splits(hvk_ctx* ctx, int q_tiles, int r_tiles, int wg_per_cu) {
 }
 
 template <typename Rows, int KP>
-int finish(hvk_ctx* ctx, const Rows& rows, const SearchBufs& b, int S, const int64_t* qn, int nq,
-           const int64_t* rn, int nr, int k, int32_t* idx, float* score, double* cosv,
-           int32_t* n_res, hipStream_t st) {
+int finalize(hvk_ctx* ctx, const Rows& rows, const SearchBufs& b, int S, const int64_t* qn, int nq,
+             const int64_t* rn, int nr, int k, int32_t* idx, float* score, double* cosv,
+             hipStream_t st) {
     hipLaunchKernelGGL((finalize_kernel<KP, Rows>), dim3((nq + 3) / 4), dim3(256), 0, st, rows,
-                       b.part, S, nq, qn, rn, nr, k, idx, score, cosv, b.flag_list, b.flag_count);
+                       b.part, S, nq, qn, rn, nr, k, idx, score, cosv, b.flag_list, b.flag_count,
+                       nullptr, nullptr, nq);
+    HVK_HIP(hipGetLastError());
+    return HVK_OK;
+}
+
+// Exact FP64 brute force of the queries left in (list, count); reports the
+// number of queries that took any exact pass.
+template <typename Rows>
+int rescan(hvk_ctx* ctx, const Rows& rows, const int32_t* list, const int32_t* count,
+           const SearchBufs& b, const int64_t* qn, const int64_t* rn, int nr, int k, int32_t* idx,
+           float* score, double* cosv, int32_t* n_res, hipStream_t st) {
     hipLaunchKernelGGL((rescan_kernel<Rows>), dim3(ctx->n_cu), dim3(256), 0, st, rows, qn, rn, nr,
-                       k, b.flag_list, b.flag_count, idx, score, cosv);
+                       k, list, count, idx, score, cosv);
     HVK_HIP(hipGetLastError());
     if (n_res)
         HVK_HIP(hipMemcpyAsync(n_res, b.flag_count, sizeof(int32_t), hipMemcpyDeviceToDevice, st));
     return HVK_OK;
+}
+
+template <typename Rows, int KP>
+int finish(hvk_ctx* ctx, const Rows& rows, const SearchBufs& b, int S, const int64_t* qn, int nq,
+           const int64_t* rn, int nr, int k, int32_t* idx, float* score, double* cosv,
+           int32_t* n_res, hipStream_t st) {
+    if (int rc = finalize<Rows, KP>(ctx, rows, b, S, qn, nq, rn, nr, k, idx, score, cosv, st))
+        return rc;
+    return rescan(ctx, rows, b.flag_list, b.flag_count, b, qn, rn, nr, k, idx, score, cosv, n_res,
+                  st);
 }
 
 template <int KP>
@@ -747,7 +844,7 @@ int run_plain(hvk_ctx* ctx, const int16_t* q, const int64_t* qn, int nq, const i
     if (rc) return rc;
     const int nmax = nq > nr ? nq : nr;
     hipLaunchKernelGGL(inv_norm_kernel, dim3((nmax + 255) / 256), dim3(256), 0, st, qn, nq, rn, nr,
-                       b.iq, b.ir, b.flag_count);
+                       b.iq, b.ir, b.flag_count, BoundRows{}, nullptr, nullptr);
     timing_mark(ctx, 1, false, st);
     hipLaunchKernelGGL((plain_kernel<KP>), dim3(q_tiles, S_used), dim3(256), 0, st, q, b.iq, nq, r,
                        b.ir, nr, ld, split_tiles, b.part);
@@ -771,15 +868,74 @@ int run_mfma(hvk_ctx* ctx, const int8_t* qlo, const int8_t* qhi, const int64_t* 
     if (rc) return rc;
     const int nmax = nq > nr ? nq : nr;
     hipLaunchKernelGGL(inv_norm_kernel, dim3((nmax + 255) / 256), dim3(256), 0, st, qn, nq, rn, nr,
-                       b.iq, b.ir, b.flag_count);
+                       b.iq, b.ir, b.flag_count, BoundRows{}, nullptr, nullptr);
     timing_mark(ctx, 1, false, st);
     hipLaunchKernelGGL((mfma_kernel<QL, RL, KP>), dim3(q_tiles, S_used), dim3(kMfmaThreads), 0, st,
-                       qlo, qhi, b.iq, nq, rlo, rhi, b.ir, nr, ld, split_tiles, b.part);
+                       qlo, qhi, b.iq, nq, rlo, rhi, b.ir, nr, ld, split_tiles, b.part, nullptr,
+                       nullptr, nullptr, nullptr);
     timing_mark(ctx, 1, true, st);
     HVK_HIP(hipGetLastError());
     return finish<RowsLimb, KP>(ctx, RowsLimb{qlo, QL > 1 ? qhi : nullptr, rlo,
                                               RL > 1 ? rhi : nullptr, ld},
                                 b, S_used, qn, nq, rn, nr, k, idx, score, cosv, n_res, st);
+}
+
+template <int KP>
+int run_mfma_bounded(hvk_ctx* ctx, const hvk_bundles& q, const hvk_bundles& r, int64_t ld, int k,
+                     int32_t* idx, float* score, double* cosv, int32_t* n_res, hipStream_t st) {
+    const int nq = q.n, nr = r.n;
+    const int q_tiles = (nq + mfma_mt<1>() - 1) / mfma_mt<1>();
+    const int r_tiles = (nr + kMfmaNT - 1) / kMfmaNT;
+    const int S = choose_splits(ctx, q_tiles, r_tiles, 2);
+    const int split_tiles = (r_tiles + S - 1) / S;
+    const int S_used = (r_tiles + split_tiles - 1) / split_tiles;
+    SearchBufs b;
+    int rc_ = prepare(ctx, nq, nr, S_used, KP, b);
+    if (rc_) return rc_;
+    const int nmax = nq > nr ? nq : nr;
+    hipLaunchKernelGGL(inv_norm_kernel, dim3((nmax + 255) / 256), dim3(256), 0, st, q.sqnorm, nq,
+                       r.sqnorm, nr, b.iq, b.ir, b.flag_count,
+                       BoundRows{q.maxabs, q.resid1, r.maxabs, r.resid1}, b.bq, b.br);
+    timing_mark(ctx, 1, false, st);
+    hipLaunchKernelGGL((mfma_kernel<1, 1, KP, true>), dim3(q_tiles, S_used), dim3(kMfmaThreads), 0,
+                       st, q.clamp8, nullptr, b.iq, nq, r.clamp8, nullptr, b.ir, nr, ld,
+                       split_tiles, b.part, b.bq, b.br, nullptr, nullptr);
+    timing_mark(ctx, 1, true, st);
+    HVK_HIP(hipGetLastError());
+    const RowsLimb rows{q.lo, q.hi, r.lo, r.hi, ld};
+    if (int rc = finalize<RowsLimb, KP>(ctx, rows, b, S_used, q.sqnorm, nq, r.sqnorm, nr, k, idx,
+                                        score, cosv, st))
+        return rc;
+    // exact second pass over the queued queries: two-limb MFMA, KP = 32
+    const bool q2 = q.hi != nullptr, r2 = r.hi != nullptr;
+    const int cap = nq < kPass2Cap ? nq : kPass2Cap;
+    const int tiles2 = (cap + mfma_mt<2>() - 1) / mfma_mt<2>();
+    const int split2 = (r_tiles + kPass2Splits - 1) / kPass2Splits;
+    const int S2 = (r_tiles + split2 - 1) / split2;
+    const dim3 g2(tiles2, S2), blk(kMfmaThreads);
+    if (q2 && r2)
+        hipLaunchKernelGGL((mfma_kernel<2, 2, kPass2KP>), g2, blk, 0, st, q.lo, q.hi, b.iq, cap,
+                           r.lo, r.hi, b.ir, nr, ld, split2, b.part2, nullptr, nullptr,
+                           b.flag_list, b.flag_count);
+    else if (q2)
+        hipLaunchKernelGGL((mfma_kernel<2, 1, kPass2KP>), g2, blk, 0, st, q.lo, q.hi, b.iq, cap,
+                           r.lo, nullptr, b.ir, nr, ld, split2, b.part2, nullptr, nullptr,
+                           b.flag_list, b.flag_count);
+    else if (r2)
+        hipLaunchKernelGGL((mfma_kernel<1, 2, kPass2KP>), g2, blk, 0, st, q.lo, nullptr, b.iq, cap,
+                           r.lo, r.hi, b.ir, nr, ld, split2, b.part2, nullptr, nullptr,
+                           b.flag_list, b.flag_count);
+    else
+        hipLaunchKernelGGL((mfma_kernel<1, 1, kPass2KP>), g2, blk, 0, st, q.lo, nullptr, b.iq, cap,
+                           r.lo, nullptr, b.ir, nr, ld, split2, b.part2, nullptr, nullptr,
+                           b.flag_list, b.flag_count);
+    HVK_HIP(hipGetLastError());
+    hipLaunchKernelGGL((finalize_kernel<kPass2KP, RowsLimb>), dim3((nq + 3) / 4), dim3(256), 0, st,
+                       rows, b.part2, S2, nq, q.sqnorm, r.sqnorm, nr, k, idx, score, cosv,
+                       b.flag_list2, b.flag_count2, b.flag_list, b.flag_count, cap);
+    HVK_HIP(hipGetLastError());
+    return rescan(ctx, rows, b.flag_list2, b.flag_count2, b, q.sqnorm, r.sqnorm, nr, k, idx, score,
+                  cosv, n_res, st);
 }
 
 template <int QL, int RL>
@@ -838,4 +994,27 @@ extern "C" int hvk_search_mfma(hvk_ctx* ctx, const int8_t* d_q_lo, const int8_t*
     if (!q2 && r2) return hvk::dispatch_mfma_kp<1, 2>(ctx, kp, d_q_lo, nullptr, d_q_sqnorm, n_q, d_r_lo, d_r_hi, d_r_sqnorm, n_r, ld, k, d_idx, d_score, d_cos64, d_n_rescored, st);
     if (q2 && !r2) return hvk::dispatch_mfma_kp<2, 1>(ctx, kp, d_q_lo, d_q_hi, d_q_sqnorm, n_q, d_r_lo, nullptr, d_r_sqnorm, n_r, ld, k, d_idx, d_score, d_cos64, d_n_rescored, st);
     return hvk::dispatch_mfma_kp<2, 2>(ctx, kp, d_q_lo, d_q_hi, d_q_sqnorm, n_q, d_r_lo, d_r_hi, d_r_sqnorm, n_r, ld, k, d_idx, d_score, d_cos64, d_n_rescored, st);
+}
+
+extern "C" int hvk_search_mfma_bounded(hvk_ctx* ctx, const hvk_bundles* q, const hvk_bundles* r,
+                                       int64_t ld, int k, int32_t* d_idx, float* d_score,
+                                       double* d_cos64, int32_t* d_n_rescored, hvk_stream stream) {
+    if (!ctx || !q || !r || !d_idx || !d_score) return HVK_ERR_ARG;
+    for (const hvk_bundles* x : {q, r})
+        if (!x->clamp8 || !x->lo || !x->sqnorm || !x->maxabs || !x->resid1 || x->n < 0)
+            return HVK_ERR_ARG;
+    if (k < 1 || k > 29) return HVK_ERR_ARG;
+    if (ld == 0) ld = ctx->ld;
+    if (ld < ctx->dim || ld % hvk::kMfmaKS) return HVK_ERR_ARG;
+    if (q->n == 0) return HVK_OK;
+    if (r->n == 0) return HVK_ERR_ARG;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    hipStream_t st = (hipStream_t)stream;
+    // the bounded keys are upper bounds: a longer candidate list keeps the
+    // margin check conclusive for more queries
+    switch (hvk::pick_kp(k) * 2 > 32 ? 32 : hvk::pick_kp(k) * 2) {
+        case 8: return hvk::run_mfma_bounded<8>(ctx, *q, *r, ld, k, d_idx, d_score, d_cos64, d_n_rescored, st);
+        case 16: return hvk::run_mfma_bounded<16>(ctx, *q, *r, ld, k, d_idx, d_score, d_cos64, d_n_rescored, st);
+        default: return hvk::run_mfma_bounded<32>(ctx, *q, *r, ld, k, d_idx, d_score, d_cos64, d_n_rescored, st);
+    }
 }

@@ -27,7 +27,8 @@ EXPORTS = [
     "hvk_status_string", "hvk_last_error", "hvk_ctx_create", "hvk_ctx_destroy",
     "hvk_ctx_info", "hvk_base_hv", "hvk_pack_words", "hvk_pack", "hvk_encode",
     "hvk_search_plain", "hvk_search_mfma", "hvk_gen_size", "hvk_gen_fill",
-    "hvk_ctx_set_timing", "hvk_ctx_last_timing",
+    "hvk_ctx_set_timing", "hvk_ctx_last_timing", "hvk_encode_ex", "hvk_encode_windows",
+    "hvk_search_mfma_bounded",
 ]
 
 
@@ -38,6 +39,17 @@ class HvkError(RuntimeError):
             msg += f" ({lib.hvk_status_string(status).decode()}; {lib.hvk_last_error().decode()})"
         super().__init__(msg)
         self.status = status
+
+
+class EncodeOut(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in ("counts", "lo", "hi", "clamp8")] + [
+        ("ld", ctypes.c_int64)] + [(n, ctypes.c_void_p) for n in ("sqnorm", "maxabs", "nkmers",
+                                                                  "resid1")]
+
+
+class Bundles(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in ("clamp8", "lo", "hi", "sqnorm", "maxabs",
+                                               "resid1")] + [("n", ctypes.c_int)]
 
 
 class GenSpec(ctypes.Structure):
@@ -87,6 +99,10 @@ def lib():
                                    vp, vp, vp, vp, vp]
     L.hvk_search_mfma.argtypes = [vp, vp, vp, vp, i32, vp, vp, vp, i32, i64, i32,
                                   vp, vp, vp, vp, vp]
+    L.hvk_encode_ex.argtypes = [vp, vp, vp, vp, i32, i32, ctypes.POINTER(EncodeOut), vp]
+    L.hvk_encode_windows.argtypes = [vp, vp, vp, vp, i32, i32, ctypes.POINTER(EncodeOut), vp]
+    L.hvk_search_mfma_bounded.argtypes = [vp, ctypes.POINTER(Bundles), ctypes.POINTER(Bundles),
+                                          i64, i32, vp, vp, vp, vp, vp]
     L.hvk_ctx_set_timing.argtypes = [vp, i32]
     L.hvk_ctx_last_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
     L.hvk_gen_size.argtypes = [ctypes.POINTER(GenSpec), ctypes.POINTER(i64), ctypes.POINTER(i64)]
@@ -224,6 +240,29 @@ class Encoded:
     sqnorm: object   # torch int64 [n]
     maxabs: object   # torch int32 [n]
     nkmers: object   # torch int32 [n]
+    clamp8: object = None  # torch int8 [n][ld] or None
+    resid1: object = None  # torch int32 [n] or None
+
+
+def _alloc(n, ld, counts, limbs, hi, clamp):
+    torch = _torch()
+    dev = "cuda"
+    return Encoded(
+        n,
+        torch.empty((n, ld), dtype=torch.int16, device=dev) if counts else None,
+        torch.empty((n, ld), dtype=torch.int8, device=dev) if limbs else None,
+        torch.empty((n, ld), dtype=torch.int8, device=dev) if (limbs and hi) else None,
+        torch.empty(n, dtype=torch.int64, device=dev),
+        torch.empty(n, dtype=torch.int32, device=dev),
+        torch.empty(n, dtype=torch.int32, device=dev),
+        torch.empty((n, ld), dtype=torch.int8, device=dev) if clamp else None,
+        torch.empty(n, dtype=torch.int32, device=dev) if clamp else None)
+
+
+def _out_struct(e: Encoded, ld: int) -> EncodeOut:
+    p = lambda t: None if t is None else t.data_ptr()
+    return EncodeOut(p(e.counts), p(e.lo), p(e.hi), p(e.clamp8), ld, p(e.sqnorm), p(e.maxabs),
+                     p(e.nkmers), p(e.resid1))
 
 
 class DeviceSeqs:
@@ -240,20 +279,27 @@ class DeviceSeqs:
         self.offsets = torch.from_numpy(np.ascontiguousarray(offsets, np.int64)).cuda()
 
 
-def encode(ctx: Context, seqs: DeviceSeqs, counts=True, limbs=True, hi=True) -> Encoded:
+def encode(ctx: Context, seqs: DeviceSeqs, counts=True, limbs=True, hi=True,
+           clamp=False) -> Encoded:
+    e = _alloc(seqs.n, ctx.ld, counts, limbs, hi, clamp)
+    o = _out_struct(e, ctx.ld)
+    check(lib().hvk_encode_ex(ctx.h, _ptr(seqs.packed), _ptr(seqs.invalid), _ptr(seqs.offsets),
+                              seqs.n, seqs.max_len, ctypes.byref(o), _stream()), "hvk_encode_ex")
+    return e
+
+
+def encode_windows(ctx: Context, seqs: DeviceSeqs, starts: np.ndarray, window_len: int,
+                   counts=True, limbs=True, hi=True, clamp=False) -> Encoded:
+    """Bundles of the windows [starts[i], starts[i] + window_len) of the packed buffer."""
     torch = _torch()
-    n, ld = seqs.n, ctx.ld
-    dev = "cuda"
-    c = torch.empty((n, ld), dtype=torch.int16, device=dev) if counts else None
-    lo = torch.empty((n, ld), dtype=torch.int8, device=dev) if limbs else None
-    h = torch.empty((n, ld), dtype=torch.int8, device=dev) if (limbs and hi) else None
-    sq = torch.empty(n, dtype=torch.int64, device=dev)
-    mx = torch.empty(n, dtype=torch.int32, device=dev)
-    nk = torch.empty(n, dtype=torch.int32, device=dev)
-    check(lib().hvk_encode(ctx.h, _ptr(seqs.packed), _ptr(seqs.invalid), _ptr(seqs.offsets), n,
-                           seqs.max_len, _ptr(c), _ptr(lo), _ptr(h), ld, _ptr(sq), _ptr(mx),
-                           _ptr(nk), _stream()), "hvk_encode")
-    return Encoded(n, c, lo, h, sq, mx, nk)
+    st = torch.from_numpy(np.ascontiguousarray(starts, np.int64)).cuda()
+    e = _alloc(len(starts), ctx.ld, counts, limbs, hi, clamp)
+    o = _out_struct(e, ctx.ld)
+    check(lib().hvk_encode_windows(ctx.h, _ptr(seqs.packed), _ptr(seqs.invalid), _ptr(st),
+                                   len(starts), window_len, ctypes.byref(o), _stream()),
+          "hvk_encode_windows")
+    torch.cuda.current_stream().synchronize()  # st is released on return
+    return e
 
 
 @dataclass
@@ -296,4 +342,24 @@ def search_mfma(ctx: Context, q: Encoded, r: Encoded, k: int, q_two_limbs=None, 
                                 _ptr(r.hi) if r_two_limbs else None, _ptr(r.sqnorm), r.n, ctx.ld,
                                 k, _ptr(o.idx), _ptr(o.score), _ptr(o.cos64), _ptr(o.n_rescored),
                                 _stream()), "hvk_search_mfma")
+    return o
+
+
+def bundles(e: Encoded, two_limbs: bool | None = None) -> Bundles:
+    p = lambda t: None if t is None else t.data_ptr()
+    if two_limbs is None:
+        two_limbs = int(e.maxabs.max().item()) > 127 if e.n else False
+    return Bundles(p(e.clamp8), p(e.lo), p(e.hi) if two_limbs else None, p(e.sqnorm),
+                   p(e.maxabs), p(e.resid1), e.n)
+
+
+def search_mfma_bounded(ctx: Context, q: Encoded, r: Encoded, k: int,
+                        out: TopK | None = None, q_two_limbs=None, r_two_limbs=None) -> TopK:
+    """One-limb bounded MFMA form; q and r need clamp8/resid1
+    (encode(..., clamp=True)) and their limbs."""
+    o = out or _topk_bufs(q.n, k)
+    bq, br = bundles(q, q_two_limbs), bundles(r, r_two_limbs)
+    check(lib().hvk_search_mfma_bounded(ctx.h, ctypes.byref(bq), ctypes.byref(br), ctx.ld, k,
+                                        _ptr(o.idx), _ptr(o.score), _ptr(o.cos64),
+                                        _ptr(o.n_rescored), _stream()), "hvk_search_mfma_bounded")
     return o

@@ -38,6 +38,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-two-limb", action="store_true")
     return ap.parse_args()
 
 
@@ -66,7 +67,7 @@ def main():
     ref_seqs = H.DeviceSeqs(w.refs, w.ref_offsets)
     torch.cuda.synchronize()
     t1 = time.time()
-    db = H.encode(ctx, ref_seqs, counts=True, limbs=True, hi=True)
+    db = H.encode(ctx, ref_seqs, counts=True, limbs=True, hi=True, clamp=True)
     torch.cuda.synchronize()
     db_encode_s = time.time() - t1
     db_two_limbs = int(db.maxabs.max().item()) > 127
@@ -76,52 +77,66 @@ def main():
           file=sys.stderr, flush=True)
 
     nq, ld = a.reads, ctx.ld
-    q_lo = torch.empty((nq, ld), dtype=torch.int8, device="cuda")
-    q_hi = torch.empty((nq, ld), dtype=torch.int8, device="cuda")
-    q_sq = torch.empty(nq, dtype=torch.int64, device="cuda")
-    q_mx = torch.empty(nq, dtype=torch.int32, device="cuda")
-    q_nk = torch.empty(nq, dtype=torch.int32, device="cuda")
-    out = H._topk_bufs(nq, TOPK)
     L = H.lib()
+    db_bundles = H.bundles(db, db_two_limbs)
+    reads = H._alloc(nq, ld, counts=False, limbs=True, hi=True, clamp=True)
+    reads_out = H._out_struct(reads, ld)
+    out = H._topk_bufs(nq, TOPK)
 
-    def step():
-        H.check(L.hvk_encode(ctx.h, H._ptr(read_seqs.packed), H._ptr(read_seqs.invalid),
-                             H._ptr(read_seqs.offsets), nq, read_seqs.max_len, None,
-                             H._ptr(q_lo), H._ptr(q_hi), ld, H._ptr(q_sq), H._ptr(q_mx),
-                             H._ptr(q_nk), H._stream()), "hvk_encode")
-        enc = H.Encoded(nq, None, q_lo, q_hi, q_sq, q_mx, q_nk)
-        H.search_mfma(ctx, enc, db, TOPK, q_two_limbs=q_two, r_two_limbs=db_two_limbs, out=out)
+    def encode_reads():
+        H.check(L.hvk_encode_ex(ctx.h, H._ptr(read_seqs.packed), H._ptr(read_seqs.invalid),
+                                H._ptr(read_seqs.offsets), nq, read_seqs.max_len,
+                                H.ctypes.byref(reads_out), H._stream()), "hvk_encode_ex")
 
-    # the limb count of the read side is fixed by the data: decide once
-    H.check(L.hvk_encode(ctx.h, H._ptr(read_seqs.packed), H._ptr(read_seqs.invalid),
-                         H._ptr(read_seqs.offsets), nq, read_seqs.max_len, None, H._ptr(q_lo),
-                         H._ptr(q_hi), ld, H._ptr(q_sq), H._ptr(q_mx), H._ptr(q_nk),
-                         H._stream()), "hvk_encode")
+    encode_reads()
     torch.cuda.synchronize()
-    q_two = int(q_mx.max().item()) > 127
-    for i in range(a.warmup):
-        step()
+    q_two = int(reads.maxabs.max().item()) > 127
+    q_bundles = H.bundles(reads, q_two)
+
+    def step_bounded():
+        encode_reads()
+        H.check(L.hvk_search_mfma_bounded(ctx.h, H.ctypes.byref(q_bundles),
+                                          H.ctypes.byref(db_bundles), ld, TOPK, H._ptr(out.idx),
+                                          H._ptr(out.score), H._ptr(out.cos64),
+                                          H._ptr(out.n_rescored), H._stream()),
+                "hvk_search_mfma_bounded")
+
+    def step_two_limb():
+        encode_reads()
+        H.search_mfma(ctx, reads, db, TOPK, q_two_limbs=q_two, r_two_limbs=db_two_limbs, out=out2)
+
+    def timed(step, label, steps, warmup):
+        for i in range(warmup):
+            step()
+            torch.cuda.synchronize()
+            print(f"[bench] {label} warmup {i} done", file=sys.stderr, flush=True)
+        enc_ms, srch_ms = [], []
         torch.cuda.synchronize()
-        print(f"[bench] warmup {i} done", file=sys.stderr, flush=True)
-
-    enc_ms, srch_ms, step_s = [], [], []
-    torch.cuda.synchronize()
-    t_start = time.time()
-    for i in range(a.steps):
-        ts = time.time()
-        step()
+        t_start = time.time()
+        for i in range(steps):
+            ts = time.time()
+            step()
+            torch.cuda.synchronize()
+            e, s_ = ctx.last_timing()
+            enc_ms.append(e)
+            srch_ms.append(s_)
+            print(f"[bench] {label} step {i}: {time.time() - ts:.3f}s encode {e:.1f}ms "
+                  f"search {s_:.1f}ms", file=sys.stderr, flush=True)
         torch.cuda.synchronize()
-        step_s.append(time.time() - ts)
-        e, s = ctx.last_timing()
-        enc_ms.append(e)
-        srch_ms.append(s)
-        print(f"[bench] step {i}: {step_s[-1]:.3f}s encode {e:.1f}ms search {s:.1f}ms",
-              file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    total = time.time() - t_start
-    ms_per_step = 1000.0 * total / a.steps
-    value = nq / (total / a.steps)
+        return (time.time() - t_start) / steps, enc_ms, srch_ms
+
+    sec_step, enc_ms, srch_ms = timed(step_bounded, "mfma-1limb-bounded", a.steps, a.warmup)
+    ms_per_step = 1000.0 * sec_step
+    value = nq / sec_step
     n_res = int(out.n_rescored.item())
+
+    # the two-limb MFMA form on the same reads (same results required)
+    out2 = H._topk_bufs(nq, TOPK)
+    two = None
+    if not a.no_two_limb:
+        sec2, _, srch2 = timed(step_two_limb, "mfma-2limb", 1, 1)
+        two = {"reads_per_s": nq / sec2, "kernel_ms": float(np.mean(srch2)),
+               "top5_equal": bool(torch.equal(out2.idx, out.idx))}
 
     # roofline of the dominant kernel (the MFMA search)
     s_kernel = float(np.mean(srch_ms)) / 1000.0
@@ -129,7 +144,8 @@ def main():
     achieved_tops = algo_ops / s_kernel / 1e12
     e_kernel = float(np.mean(enc_ms)) / 1000.0
     lens = np.diff(w.read_offsets)
-    enc_bytes = float(np.sum((lens + 3) // 4)) + nq * (2.0 * ld) + nq * 8  # in + limbs + norm
+    # packed input + lo/hi/clamped rows + per-row norm, max, residual, k-mer count
+    enc_bytes = float(np.sum((lens + 3) // 4)) + nq * (3.0 * ld) + nq * 20
     enc_gbs = enc_bytes / e_kernel / 1e9
 
     # plain form on a subset
@@ -197,13 +213,15 @@ def main():
                                f"(2,000 ancestors x 50, 1,300-1,600 nt, 2-8% div, 1% indel), "
                                f"{nq} reads 150-300 nt, 0.5-2% error, top-{TOPK}",
                    "reads": nq, "refs": n_refs, "dim": DIM, "k": KMER, "topk": TOPK,
+                   "form": "MFMA, one clamped int8 limb per side + exact bound, FP64 re-score",
                    "read_limbs": 2 if q_two else 1, "ref_limbs": 2 if db_two_limbs else 1},
         "roofline": {"bound": "mfma", "achieved": achieved_tops, "peak": MFMA_INT8_PEAK_TOPS,
                      "unit": "TFLOP/s", "frac": achieved_tops / MFMA_INT8_PEAK_TOPS,
-                     "traffic": None, "kernel": "mfma_kernel",
+                     "traffic": None, "kernel": "mfma_kernel<1,1,8,true>",
                      "kernel_ms": float(np.mean(srch_ms))},
         "encode": {"kernel_ms": float(np.mean(enc_ms)), "achieved_gbs": enc_gbs,
                    "frac_hbm": enc_gbs / HBM_PEAK_GBS},
+        "mfma_two_limb": two,
         "plain": plain,
         "mfma_vs_plain": (value / plain["reads_per_s"]) if plain else None,
         "n_rescored": n_res,

@@ -104,6 +104,42 @@ int hvk_encode(hvk_ctx* ctx, const uint32_t* d_packed, const uint32_t* d_invalid
                int64_t* d_sqnorm, int32_t* d_maxabs, int32_t* d_nkmers,
                hvk_stream stream);
 
+/* Outputs of the encoder (every pointer nullable; rows of ld elements,
+ * ld = 0 -> ctx row_ld; padding columns [D, ld) are written as zero):
+ *   counts  int16 [n][ld]  counts (int16 wrap as the reference's HV_16)
+ *   lo, hi  int8  [n][ld]  limbs: count = 256*hi + lo, lo in [-128,127]
+ *   clamp8  int8  [n][ld]  count clamped to [-127, 127]
+ *   sqnorm  int64 [n]      exact sum of count^2
+ *   maxabs  int32 [n]      max |count|
+ *   nkmers  int32 [n]      number of k-mers bundled
+ *   resid1  int32 [n]      sum of |count - clamp8| (L1 norm of the residual) */
+typedef struct {
+    int16_t* counts;
+    int8_t* lo;
+    int8_t* hi;
+    int8_t* clamp8;
+    int64_t ld;
+    int64_t* sqnorm;
+    int32_t* maxabs;
+    int32_t* nkmers;
+    int32_t* resid1;
+} hvk_encode_out;
+
+/* hvk_encode with every output of hvk_encode_out. */
+int hvk_encode_ex(hvk_ctx* ctx, const uint32_t* d_packed, const uint32_t* d_invalid,
+                  const int64_t* d_offsets, int n_seqs, int max_len,
+                  const hvk_encode_out* out, hvk_stream stream);
+
+/* Windowed encoding: one bundle per window of window_len bases starting at
+ * base offset d_starts[i] of the packed sequence buffer, each with
+ * HVCache::compute semantics of the window's substring.  Replaces the
+ * reference's windowed GPU encoder hdc_encode / launch_hdc_encode
+ * (src/encode.cu:69-165) and the exact windows of HVCache::precompute
+ * (src/encoder.cpp:134-196). */
+int hvk_encode_windows(hvk_ctx* ctx, const uint32_t* d_packed, const uint32_t* d_invalid,
+                       const int64_t* d_starts, int n_windows, int window_len,
+                       const hvk_encode_out* out, hvk_stream stream);
+
 /* ---- search -----------------------------------------------------------
  * Top-k cosine search of n_q queries against n_r references.  The order is
  * (c desc, reference index asc) with c = dot / (sqrt(|q|^2) sqrt(|r|^2))
@@ -141,6 +177,29 @@ int hvk_search_mfma(hvk_ctx* ctx,
                     int64_t ld, int k,
                     int32_t* d_idx, float* d_score, double* d_cos64,
                     int32_t* d_n_rescored, hvk_stream stream);
+
+/* One side of a bounded search: rows as hvk_encode_ex writes them. */
+typedef struct {
+    const int8_t* clamp8;   /* [n][ld] counts clamped to [-127, 127] */
+    const int8_t* lo;       /* [n][ld] exact limbs, value = 256*hi + lo */
+    const int8_t* hi;       /* NULL when every |count| <= 127 */
+    const int64_t* sqnorm;  /* [n] */
+    const int32_t* maxabs;  /* [n] */
+    const int32_t* resid1;  /* [n] sum |count - clamp8| */
+    int n;
+} hvk_bundles;
+
+/* MFMA form with one limb per side and an exact bound (same results as
+ * hvk_search_mfma).  The matrix cores multiply clamped counts (one int8 limb
+ * each side, half the work of the two-limb form); for each pair the FP32
+ * score is raised by an upper bound on what the clamps removed,
+ *   |dot - dot_clamped| <= min(max|q|,127) resid1(r) + resid1(q) max|r|,
+ * so the candidates kept are those whose upper bound ranks high.  Their exact
+ * dots are recomputed from the limbs and re-scored in FP64, with the same
+ * margin check and exact re-scan as hvk_search_mfma. */
+int hvk_search_mfma_bounded(hvk_ctx* ctx, const hvk_bundles* q, const hvk_bundles* r, int64_t ld,
+                            int k, int32_t* d_idx, float* d_score, double* d_cos64,
+                            int32_t* d_n_rescored, hvk_stream stream);
 
 /* ---- synthetic workloads (host, deterministic) -------------------------
  * Ancestors of uniform ACGT, descendants by substitution + indels, and reads

@@ -76,3 +76,40 @@ def test_base_hv_matches_oracle():
         ctx = H.Context(dim, 5)
         for ch in "ACGTN":
             assert np.array_equal(ctx.base_hv(ch), orc.base_hv(dim, ch))
+
+
+@pytest.mark.parametrize("dim,k", [(10240, 12), (10000, 5)])
+def test_clamp_and_residual_outputs(dim, k):
+    """clamp8 = clip(count, -127, 127); resid1 = sum |count - clamp8|."""
+    rng = np.random.default_rng(21)
+    seqs = [seqgen.random_seq(rng, 1500), "A" * 400, seqgen.random_seq(rng, 200), "ACGTAC" * 100]
+    ctx, enc = _encode(dim, k, seqs, clamp=True)
+    c = enc.counts.cpu().numpy().astype(np.int32)[:, :dim]
+    cl = enc.clamp8.cpu().numpy().astype(np.int32)
+    want = np.clip(c, -127, 127)
+    assert np.array_equal(cl[:, :dim], want)
+    assert not cl[:, dim:].any()
+    assert np.array_equal(enc.resid1.cpu().numpy(), np.abs(c - want).sum(1))
+    assert enc.resid1.cpu().numpy()[1] > 0  # poly-A saturates
+
+
+@pytest.mark.parametrize("dim,k", [(10000, 5), (10240, 12)])
+def test_windows_match_substring_compute(dim, k):
+    """hvk_encode_windows == compute() of each window's substring."""
+    import hdc_amd as H
+    import torch
+    seqs = seqgen.mixed_set(55, 5, 300, 700)
+    buf, off = H.concat(seqs)
+    ctx = H.Context(dim, k)
+    dev = H.DeviceSeqs(buf, off)
+    starts = []
+    for i, s in enumerate(seqs):
+        starts += [int(off[i]) + x for x in range(0, len(s) - 125 + 1, 10)]
+    starts = np.array(starts[:200], np.int64)
+    enc = H.encode_windows(ctx, dev, starts, 125)
+    torch.cuda.synchronize()
+    got = enc.counts.cpu().numpy()[:, :dim]
+    text = buf.decode()
+    for j in range(0, len(starts), 7):
+        ref, _ = orc.compute(dim, k, text[starts[j]:starts[j] + 125])
+        assert np.array_equal(got[j], ref), j

@@ -21,10 +21,25 @@ def encoded_from_counts(counts, ld):
     v = c.astype(np.int32)
     lo = ((v + 128) % 256 - 128).astype(np.int8)
     hi = ((v - lo) // 256).astype(np.int8)
+    cl = np.clip(v, -127, 127)
     sq = (c.astype(np.int64) ** 2).sum(1)
     mx = np.abs(v).max(1).astype(np.int32) if n else np.zeros(0, np.int32)
+    res = np.abs(v - cl).sum(1).astype(np.int32)
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
-    return H.Encoded(n, t(c), t(lo), t(hi), t(sq), t(mx), t(np.zeros(n, np.int32)))
+    return H.Encoded(n, t(c), t(lo), t(hi), t(sq), t(mx), t(np.zeros(n, np.int32)),
+                     t(cl.astype(np.int8)), t(res))
+
+
+FORMS = ["plain", "mfma", "bounded"]
+
+
+def run_form(form, ctx, q, r, k):
+    import hdc_amd as H
+    if form == "plain":
+        return H.search_plain(ctx, q, r, k)
+    if form == "mfma":
+        return H.search_mfma(ctx, q, r, k)
+    return H.search_mfma_bounded(ctx, q, r, k)
 
 
 @pytest.fixture(scope="module")
@@ -36,8 +51,8 @@ def workload():
                        read_len_max=300, err_min=0.005, err_max=0.02)
     w = H.generate(spec)
     ctx = H.Context(DIM, K)
-    refs = H.encode(ctx, H.DeviceSeqs(w.refs, w.ref_offsets))
-    reads = H.encode(ctx, H.DeviceSeqs(w.reads, w.read_offsets))
+    refs = H.encode(ctx, H.DeviceSeqs(w.refs, w.ref_offsets), clamp=True)
+    reads = H.encode(ctx, H.DeviceSeqs(w.reads, w.read_offsets), clamp=True)
     torch.cuda.synchronize()
     rc = refs.counts.cpu().numpy()[:, :DIM]
     qc = reads.counts.cpu().numpy()[:, :DIM]
@@ -57,29 +72,29 @@ def _check(res, ref_idx, ref_cos, k):
     assert err.max(initial=0.0) <= COS_REL_TOL
 
 
-@pytest.mark.parametrize("form", ["plain", "mfma"])
+@pytest.mark.parametrize("form", FORMS)
 @pytest.mark.parametrize("k", [1, 5, 10])
 def test_topk_matches_oracle(workload, form, k):
     import hdc_amd as H
     ctx, refs, reads, rc, qc, w = workload
     print(f"search form={form} k={k} nq={reads.n} nr={refs.n}", flush=True)
     ref_idx, ref_cos = orc.topk16(qc, rc, k)
-    res = (H.search_plain if form == "plain" else H.search_mfma)(ctx, reads, refs, k)
+    res = run_form(form, ctx, reads, refs, k)
     _check(res, ref_idx, ref_cos, k)
 
 
-@pytest.mark.parametrize("form", ["plain", "mfma"])
+@pytest.mark.parametrize("form", FORMS)
 def test_reads_find_their_source(workload, form):
     """Top-1 of a read sampled from reference r is r (or an equal-score tie)."""
     import hdc_amd as H
     ctx, refs, reads, rc, qc, w = workload
-    res = (H.search_plain if form == "plain" else H.search_mfma)(ctx, reads, refs, 5)
+    res = run_form(form, ctx, reads, refs, 5)
     idx = res.idx.cpu().numpy()
     hit = (idx[:, 0] == w.truth).mean()
     assert hit > 0.9
 
 
-@pytest.mark.parametrize("form", ["plain", "mfma"])
+@pytest.mark.parametrize("form", FORMS)
 def test_exact_ties_duplicate_references(workload, form):
     """Duplicated references tie exactly; order must be index ascending and
     the near-tie margin check must route the queries to the exact re-scan."""
@@ -90,21 +105,21 @@ def test_exact_ties_duplicate_references(workload, form):
     db = encoded_from_counts(dup, ctx.ld)
     q = encoded_from_counts(qc[:33], ctx.ld)
     ref_idx, ref_cos = orc.topk16(qc[:33], dup, 10)
-    res = (H.search_plain if form == "plain" else H.search_mfma)(ctx, q, db, 10)
+    res = run_form(form, ctx, q, db, 10)
     _check(res, ref_idx, ref_cos, 10)
-    # 21 copies tie at c = 1 for queries equal to reference 5: the tie group
-    # outgrows the FP32 candidate list (16), so those queries are re-scanned
-    big = np.concatenate([rc[:30], np.repeat(rc[5:6], 20, axis=0), rc[30:60]])
+    # 41 copies tie at c = 1 for queries equal to reference 5: the tie group
+    # outgrows every candidate list (16 / 32), so those queries go through the
+    # exact passes down to the brute-force re-scan
+    big = np.concatenate([rc[:30], np.repeat(rc[5:6], 40, axis=0), rc[30:60]])
     qn = np.concatenate([rc[5:6], qc[:7], rc[5:6]])
     ref_idx, ref_cos = orc.topk16(qn, big, 10)
-    res = (H.search_plain if form == "plain" else H.search_mfma)(
-        ctx, encoded_from_counts(qn, ctx.ld), encoded_from_counts(big, ctx.ld), 10)
+    res = run_form(form, ctx, encoded_from_counts(qn, ctx.ld), encoded_from_counts(big, ctx.ld), 10)
     _check(res, ref_idx, ref_cos, 10)
     assert list(ref_idx[0]) == [5] + list(range(30, 39))
     assert int(res.n_rescored.cpu().item()) >= 2
 
 
-@pytest.mark.parametrize("form", ["plain", "mfma"])
+@pytest.mark.parametrize("form", FORMS)
 def test_zero_norm_rows(workload, form):
     import hdc_amd as H
     ctx, refs, reads, rc, qc, w = workload
@@ -116,18 +131,18 @@ def test_zero_norm_rows(workload, form):
     db = encoded_from_counts(db_np, ctx.ld)
     q = encoded_from_counts(q_np, ctx.ld)
     ref_idx, ref_cos = orc.topk16(q_np, db_np, 5)
-    res = (H.search_plain if form == "plain" else H.search_mfma)(ctx, q, db, 5)
+    res = run_form(form, ctx, q, db, 5)
     _check(res, ref_idx, ref_cos, 5)
     assert list(res.idx.cpu().numpy()[2]) == [0, 1, 2, 3, 4]
 
 
-@pytest.mark.parametrize("form", ["plain", "mfma"])
+@pytest.mark.parametrize("form", FORMS)
 def test_k_larger_than_db(workload, form):
     import hdc_amd as H
     ctx, refs, reads, rc, qc, w = workload
     db = encoded_from_counts(rc[:3], ctx.ld)
     q = encoded_from_counts(qc[:4], ctx.ld)
-    res = (H.search_plain if form == "plain" else H.search_mfma)(ctx, q, db, 5)
+    res = run_form(form, ctx, q, db, 5)
     idx = res.idx.cpu().numpy()
     ref_idx, _ = orc.topk16(qc[:4], rc[:3], 5)
     assert np.array_equal(idx, ref_idx)
@@ -142,14 +157,15 @@ def test_two_limb_queries(workload):
     seqs = [w.refs[w.ref_offsets[i]:w.ref_offsets[i + 1]].decode() for i in (0, 30, 77)]
     seqs += ["ACGTTGCA" * 150, "A" * 800]
     buf, off = H.concat(seqs)
-    q = H.encode(ctx, H.DeviceSeqs(buf, off))
+    q = H.encode(ctx, H.DeviceSeqs(buf, off), clamp=True)
     torch.cuda.synchronize()
     assert int(q.maxabs.max().item()) > 127
     qn = q.counts.cpu().numpy()[:, :DIM]
     ref_idx, ref_cos = orc.topk16(qn, rc, 5)
-    res = H.search_mfma(ctx, q, refs, 5)
-    _check(res, ref_idx, ref_cos, 5)
-    assert list(res.idx.cpu().numpy()[0][:1]) == [0]
+    for fn in (H.search_mfma, H.search_mfma_bounded):
+        res = fn(ctx, q, refs, 5)
+        _check(res, ref_idx, ref_cos, 5)
+        assert list(res.idx.cpu().numpy()[0][:1]) == [0]
 
 
 def test_bipolar_single_limb():
