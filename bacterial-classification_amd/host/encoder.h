@@ -1,0 +1,60 @@
+// encoder.h -- drop-in for the reference's src/encoder.h (:1-42): HVCache,
+// an LRU cache of sequence bundles, with the same members and semantics.
+// Bundles are computed on the GPU (libhdc_amd); the LRU bookkeeping is
+// host-side and follows the reference's order of operations, so cache hits,
+// evictions and the sliding-window values of precompute() match it.
+#pragma once
+#include <list>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "HV.h"
+
+constexpr size_t CACHE_BYTES = 1'000'000'000;
+constexpr size_t HV_MEM = HV_DIM * 2;
+constexpr size_t CACHE_SIZE = CACHE_BYTES / HV_MEM;
+
+struct SIM {
+    size_t pos;
+    float score;
+};
+
+class HVCache {
+public:
+    // kmers is filled when 4^channel <= 65536 (the reference always fills it;
+    // at k = 12 it would take ~23 GB); the GPU never reads it.
+    explicit HVCache(size_t max_entries = CACHE_SIZE, size_t channel = KMER_SIZE);
+
+    const HV_16& get(const std::string& seq);
+
+    // Windows of SEQUENCE_LENGTH at SIM_STEP, cached with the reference's
+    // incremental update (src/encoder.cpp:134-196).
+    void precompute(const std::string& seq);
+
+    // Best window of reference for query (src/encoder.cpp:198-215); the
+    // window cosines are evaluated on the GPU in one batch.
+    SIM similarity(const std::string& query, const std::string& reference);
+
+    // Extension: computes the bundles of seqs in one GPU batch; later get()
+    // misses on them use these values instead of a launch each.  The LRU is
+    // not touched, so its state stays the reference's.
+    void prefetch(const std::vector<std::string>& seqs);
+
+    std::unordered_map<std::string, HV> kmers;
+
+private:
+    struct Entry {
+        HV_16 hv;
+        std::list<std::string>::iterator lru_it;
+    };
+
+    size_t capacity;
+    std::list<std::string> lru;
+    std::unordered_map<std::string, Entry> map;
+    size_t channel;
+    std::unordered_map<std::string, HV_16> prefetched;
+
+    HV_16 compute(const std::string& seq);
+    void insert(const std::string& key, HV_16 hv);
+};

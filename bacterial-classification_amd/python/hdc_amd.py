@@ -28,7 +28,7 @@ EXPORTS = [
     "hvk_ctx_info", "hvk_base_hv", "hvk_pack_words", "hvk_pack", "hvk_encode",
     "hvk_search_plain", "hvk_search_mfma", "hvk_gen_size", "hvk_gen_fill",
     "hvk_ctx_set_timing", "hvk_ctx_last_timing", "hvk_encode_ex", "hvk_encode_windows",
-    "hvk_search_mfma_bounded",
+    "hvk_search_mfma_bounded", "hvk_prepare_counts",
 ]
 
 
@@ -103,6 +103,7 @@ def lib():
     L.hvk_encode_windows.argtypes = [vp, vp, vp, vp, i32, i32, ctypes.POINTER(EncodeOut), vp]
     L.hvk_search_mfma_bounded.argtypes = [vp, ctypes.POINTER(Bundles), ctypes.POINTER(Bundles),
                                           i64, i32, vp, vp, vp, vp, vp]
+    L.hvk_prepare_counts.argtypes = [vp, vp, i32, ctypes.POINTER(EncodeOut), vp]
     L.hvk_ctx_set_timing.argtypes = [vp, i32]
     L.hvk_ctx_last_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
     L.hvk_gen_size.argtypes = [ctypes.POINTER(GenSpec), ctypes.POINTER(i64), ctypes.POINTER(i64)]
@@ -363,3 +364,15 @@ def search_mfma_bounded(ctx: Context, q: Encoded, r: Encoded, k: int,
                                         _ptr(o.idx), _ptr(o.score), _ptr(o.cos64),
                                         _ptr(o.n_rescored), _stream()), "hvk_search_mfma_bounded")
     return o
+
+
+def prepare_counts(ctx: Context, counts) -> Encoded:
+    """Search operands (limbs, clamp, norms) from a torch int16 [n][ld] tensor."""
+    n = counts.shape[0]
+    e = _alloc(n, ctx.ld, counts=False, limbs=True, hi=True, clamp=True)
+    e.counts = counts
+    o = _out_struct(e, ctx.ld)
+    o.counts = None
+    check(lib().hvk_prepare_counts(ctx.h, _ptr(counts), n, ctypes.byref(o), _stream()),
+          "hvk_prepare_counts")
+    return e

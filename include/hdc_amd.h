@@ -140,6 +140,15 @@ int hvk_encode_windows(hvk_ctx* ctx, const uint32_t* d_packed, const uint32_t* d
                        const int64_t* d_starts, int n_windows, int window_len,
                        const hvk_encode_out* out, hvk_stream stream);
 
+/* Search operands from int16 count rows the caller already holds (HV_16
+ * vectors of the C++ API): fills the lo/hi/clamp8 rows and the per-row
+ * sqnorm/maxabs/resid1 of out (out->counts must be NULL; nullable outputs are
+ * skipped).  d_counts rows have stride out->ld (0 -> ctx row_ld); columns
+ * [D, ld) are treated as zero.  Replaces compute_inv_norms_kernel
+ * (src/cosine.cu:35-50). */
+int hvk_prepare_counts(hvk_ctx* ctx, const int16_t* d_counts, int n, const hvk_encode_out* out,
+                       hvk_stream stream);
+
 /* ---- search -----------------------------------------------------------
  * Top-k cosine search of n_q queries against n_r references.  The order is
  * (c desc, reference index asc) with c = dot / (sqrt(|q|^2) sqrt(|r|^2))

@@ -119,3 +119,16 @@ def test_generator_rejects_bad_spec():
     bad = H.make_spec(read_len_min=300, read_len_max=100)
     a, b = ctypes.c_int64(), ctypes.c_int64()
     assert L.hvk_gen_size(ctypes.byref(bad), ctypes.byref(a), ctypes.byref(b)) == H.HVK_ERR_ARG
+
+
+def test_cli_usage_errors_throw():
+    """The drop-in tools reject a wrong argument count as the reference does
+    (std::runtime_error with its usage text), before touching the device."""
+    import subprocess
+    b = os.path.join(ROOT, "bacterial-classification_amd", "bin")
+    if not os.path.exists(os.path.join(b, "identify")):
+        pytest.skip("tools not built")
+    p = subprocess.run([os.path.join(b, "identify"), "a"], capture_output=True, text=True)
+    assert p.returncode != 0 and "Usage: identify" in p.stderr
+    p = subprocess.run([os.path.join(b, "propogate-children")], capture_output=True, text=True)
+    assert p.returncode != 0 and "Usage: propogate-children" in p.stderr
