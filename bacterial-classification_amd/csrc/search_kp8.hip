@@ -1,0 +1,7 @@
+// search_kp8.hip -- search run functions with 8-entry candidate lists.
+#include "search_impl.h"
+
+namespace hvk {
+
+HVK_SEARCH_RUNS(8, )
+}  // namespace hvk

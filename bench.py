@@ -39,6 +39,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-two-limb", action="store_true")
+    ap.add_argument("--copy-calib-mb", type=int, default=0,
+                    help="first run a device-to-device copy of this many MiB (a kernel of known "
+                         "traffic, for scaling HBM byte counters)")
     return ap.parse_args()
 
 
@@ -52,6 +55,16 @@ def main():
     import hdc_amd as H
 
     torch.cuda.set_device(0)
+    if a.copy_calib_mb > 0:
+        nbytes = a.copy_calib_mb << 20
+        src = torch.ones(nbytes, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] copy calibration: {nbytes} bytes read and written", file=sys.stderr,
+              flush=True)
+        del src, dst
     t0 = time.time()
     spec = H.make_spec(seed=0, n_ancestors=a.ancestors, desc_per_ancestor=50, anc_len_min=1300,
                        anc_len_max=1600, div_min=0.02, div_max=0.08, indel_rate=0.01,
