@@ -143,6 +143,18 @@ int hvk_ctx_set_timing(hvk_ctx* ctx, int enable) {
     return HVK_OK;
 }
 
+int hvk_ctx_set_tuning(hvk_ctx* ctx, int param, int value) {
+    if (!ctx) return HVK_ERR_ARG;
+    switch (param) {
+        case HVK_TUNE_SPLITS:
+            if (value < 0 || value > 64) return HVK_ERR_ARG;
+            ctx->tune_splits = value;
+            return HVK_OK;
+        default:
+            return HVK_ERR_ARG;
+    }
+}
+
 int hvk_ctx_last_timing(hvk_ctx* ctx, float* encode_ms, float* search_ms) {
     if (!ctx) return HVK_ERR_ARG;
     float* out[2] = {encode_ms, search_ms};

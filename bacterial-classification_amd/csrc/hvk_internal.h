@@ -61,6 +61,8 @@ struct hvk_ctx {
     bool timing = false;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     bool ev_used[2] = {false, false};  // encode, search
+    // tuning (hvk_ctx_set_tuning); 0 = automatic
+    int tune_splits = 0;
 };
 
 // Kernel launchers (defined in the .hip files).

@@ -68,20 +68,23 @@ __device__ __forceinline__ void static_for(F&& f) {
 }
 
 // ------------------------------------------------------------------------
-// per-workgroup top lists in LDS (keys only; 0 = empty)
+// per-workgroup top lists in LDS (keys only; 0 = empty).  Query-minor
+// layout ([j][m]): the merges, one thread per query, touch consecutive
+// words.  The candidate buffer (CAP keys per query, layout [slot][m]) is
+// passed in: the MFMA kernel keeps it in LDS stages idle during its
+// epilogue.
 // ------------------------------------------------------------------------
-template <int MT, int KP, int CAP>
+template <int MT, int KP>
 struct TopLds {
-    uint64_t top[MT][KP];  // descending
-    uint64_t buf[MT][CAP];
-    uint64_t thr[MT];      // top[m][KP-1]
-    float thr_f[MT];       // its FP32 score, -inf while the list is not full
+    uint64_t top[KP][MT];  // top[j][m] descending in j
+    uint64_t thr[MT];      // top[KP-1][m] (0 while the list is not full)
+    float thr_f[MT];       // its FP32 score, -inf while thr is 0
     int cnt[MT];
     int ovf[2];
     int zero;  // always 0; an operand the compiler cannot fold
 
     __device__ void init(int tid, int nthr) {
-        for (int i = tid; i < MT * KP; i += nthr) top[i / KP][i % KP] = 0;
+        for (int i = tid; i < MT * KP; i += nthr) top[i / MT][i % MT] = 0;
         for (int i = tid; i < MT; i += nthr) { thr[i] = 0; thr_f[i] = -INFINITY; cnt[i] = 0; }
         if (tid == 0) { ovf[0] = 0; ovf[1] = 0; zero = 0; }
     }
@@ -90,43 +93,58 @@ struct TopLds {
     // threshold but the buffer was full: the caller keeps it for the next
     // pass, which runs after the merge has raised the threshold.  A key is
     // therefore offered at most once until accepted or fallen below.
-    __device__ __forceinline__ bool offer(int m, uint64_t key, int pass) {
+    template <int CAP>
+    __device__ __forceinline__ bool offer(int m, uint64_t key, int pass, uint64_t* buf) {
         if (key <= thr[m]) return false;
         const int slot = atomicAdd(&cnt[m], 1);
         if (slot < CAP) {
-            buf[m][slot] = key;
+            buf[slot * MT + m] = key;
             return false;
         }
         ovf[pass & 1] = 1;
         return true;
     }
 
-    __device__ void merge(int m) {
-        const int n = cnt[m] < CAP ? cnt[m] : CAP;
-        for (int q = 0; q < n; ++q) {
-            const uint64_t c = buf[m][q];
-            if (c <= top[m][KP - 1]) continue;
-            int p = KP - 1;
-            while (p > 0 && top[m][p - 1] < c) {
-                top[m][p] = top[m][p - 1];
-                --p;
+    // Merges query m's buffered keys into its list, held in registers while
+    // the candidates are inserted (r[p] = max(t[p], min(t[p-1], x))).
+    template <int CAP>
+    __device__ void merge(int m, const uint64_t* buf) {
+        const int c = cnt[m];
+        const int n = c < CAP ? c : CAP;
+        if (n > 0) {
+            uint64_t t[KP];
+#pragma unroll
+            for (int j = 0; j < KP; ++j) t[j] = top[j][m];
+            for (int q = 0; q < n; ++q) {
+                const uint64_t x = buf[q * MT + m];
+                if (x <= t[KP - 1]) continue;
+#pragma unroll
+                for (int p = KP - 1; p > 0; --p) {
+                    const uint64_t lo = t[p - 1] < x ? t[p - 1] : x;
+                    t[p] = t[p] > lo ? t[p] : lo;
+                }
+                t[0] = t[0] > x ? t[0] : x;
             }
-            top[m][p] = c;
+#pragma unroll
+            for (int j = 0; j < KP; ++j) top[j][m] = t[j];
+            if (t[KP - 1] > thr[m]) {
+                thr[m] = t[KP - 1];
+                thr_f[m] = f32_of_ord((uint32_t)(t[KP - 1] >> 32));
+            }
         }
-        thr[m] = top[m][KP - 1];
-        thr_f[m] = thr[m] ? f32_of_ord((uint32_t)(thr[m] >> 32)) : -INFINITY;
         cnt[m] = 0;
     }
 };
 
 // End of one offer pass: merge the buffers; true when a key was dropped
 // (the caller then runs another pass over its dropped keys).
-template <int MT, int KP, int CAP>
-__device__ __forceinline__ bool topk_pass_end(TopLds<MT, KP, CAP>& tl, int tid, int pass) {
+template <int CAP, int MT, int KP>
+__device__ __forceinline__ bool topk_pass_end(TopLds<MT, KP>& tl, const uint64_t* buf, int tid,
+                                              int pass) {
     __syncthreads();
     const int ov = tl.ovf[pass & 1];
     if (tid == 0) tl.ovf[(pass + 1) & 1] = 0;
-    if (tid < MT) tl.merge(tid);
+    if (tid < MT) tl.template merge<CAP>(tid, buf);
     __syncthreads();
     return ov != 0;
 }
@@ -157,7 +175,8 @@ plain_kernel(const int16_t* __restrict__ q, const float* __restrict__ iq, int nq
     constexpr int CAP = 16;
     __shared__ uint32_t sQ[KW][MT];
     __shared__ uint32_t sR[KW][NT + 1];
-    __shared__ TopLds<MT, KP, CAP> tl;
+    __shared__ TopLds<MT, KP> tl;
+    __shared__ uint64_t cbuf[CAP * MT];
     __shared__ float s_iq[MT];
 
     const int tid = threadIdx.x;
@@ -244,16 +263,16 @@ plain_kernel(const int16_t* __restrict__ q, const float* __restrict__ iq, int nq
                 const int n = n0 + tx + 32 * j;
                 if ((pass == 0 || (dropped & bit)) && m0 + m < nq && n < nr) {
                     const float s = __fmul_rn(__fmul_rn((float)acc[i][j], s_iq[m]), irv[j]);
-                    if (tl.offer(m, make_key(s, n), pass)) dropped |= bit;
+                    if (tl.template offer<CAP>(m, make_key(s, n), pass, cbuf)) dropped |= bit;
                     else dropped &= ~bit;
                 }
             });
-            if (!topk_pass_end(tl, tid, pass)) break;
+            if (!topk_pass_end<CAP>(tl, cbuf, tid, pass)) break;
         }
     }
     for (int i = tid; i < MT * KP; i += 256) {
-        const int m = i / KP, j = i % KP;
-        if (m0 + m < nq) part[((int64_t)split * nq + m0 + m) * KP + j] = tl.top[m][j];
+        const int j = i / MT, m = i % MT;
+        if (m0 + m < nq) part[((int64_t)split * nq + m0 + m) * KP + j] = tl.top[j][m];
     }
 }
 
@@ -300,10 +319,11 @@ __device__ __forceinline__ void load_lds16(const int8_t* g, int8_t* lds) {
 }
 
 // LDS tile rows are 64 bytes; 16-byte chunk c of row `row` is stored at
-// chunk c ^ swz_key(row).  A fragment read takes one chunk of 32
-// consecutive rows; with this key every 8 consecutive lanes (32 banks) and
-// every 16 consecutive lanes (64 banks) touch distinct bank groups.
-__device__ __forceinline__ int swz_key(int row) { return ((row >> 1) ^ (row >> 3)) & 3; }
+// chunk c ^ swz_key(row).  A fragment read (ds_read_b128) takes one chunk
+// of 32 consecutive rows.  gfx950's LDS serves a ds_read_b128 16 lanes (256
+// bytes, 64 banks) at a time; with this key the 16 rows of each group fall
+// on 16 distinct 16-byte bank groups (measured: no bank-conflict cycles).
+__device__ __forceinline__ int swz_key(int row) { return (row >> 2) & 3; }
 __device__ __forceinline__ int swz(int row, int chunk) {
     return row * 64 + ((chunk ^ swz_key(row)) << 4);
 }
@@ -312,10 +332,12 @@ template <int QL, int RL, int KP, int ST>
 struct MfmaSmem {
     static constexpr int MT = MfmaTile<QL, RL>::MT, NT = MfmaTile<QL, RL>::NT;
     int8_t q[ST][QL][MT * kMfmaKS];
-    int8_t r[ST][RL][NT * kMfmaKS];
-    TopLds<MT, KP, 8> tl;
+    int8_t r[ST][RL][NT * kMfmaKS];  // also the candidate buffer of the epilogue
+    TopLds<MT, KP> tl;
     float iq[MT];
     float2 bq[MT];
+    // candidates per query and pass that the idle reference stages hold
+    static constexpr int kCap = (int)(sizeof(r) / (MT * 8)) < 32 ? (int)(sizeof(r) / (MT * 8)) : 32;
 };
 
 // LDS stages: three when they fit beside the top lists, else two.
@@ -433,24 +455,11 @@ mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
         const int off = 16 * ((ln & 3) ^ swz_key(row));
         return (uint32_t)((uint32_t)(min(row0 + row, n_rows - 1) - row0) * (uint32_t)ld + off);
     };
-    // fragment reads: lane (r, h) = (lane % 32, lane / 32) reads 16 bytes of
-    // row r of a 32-row block, logical chunk 2 ks + h
-    int fq_off[2], fr_off[2];
-    {
-        const int r = lane & 31, h = lane >> 5;
-#pragma unroll
-        for (int ks = 0; ks < 2; ++ks) {
-            const int c = ((2 * ks + h) ^ swz_key(r)) << 4;
-            fq_off[ks] = (wm * 32 * WX + r) * 64 + c;
-            fr_off[ks] = (wn * 32 * WY + r) * 64 + c;
-        }
-    }
     __syncthreads();
 
     for (int n0 = n_begin; n0 < n_end; n0 += NT) {
-        auto issue_slab = [&](int kk, int buf) {
+        auto issue_slab = [&](int kk, int buf, int ln) {
             const int64_t k0 = (int64_t)kk * KS;
-            const int ln = opaque_lane();
 #pragma unroll
             for (int t = 0; t < QJ; ++t) {
                 const uint32_t vo = IND ? 0u : tile_off(ln, t, QJ, m0, nq);
@@ -482,8 +491,7 @@ mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
         // lane base per operand, block and limb offsets as immediates.
         // Lane (r, h) = (lane % 32, lane / 32) reads 16 bytes of row r of a
         // 32-row block, logical chunk 2 ks + h.
-        auto load_frags = [&](v4i (&fa)[QL][WX], v4i (&fb)[RL][WY], int st, int ks) {
-            const int ln = opaque_lane();
+        auto load_frags = [&](v4i (&fa)[QL][WX], v4i (&fb)[RL][WY], int st, int ks, int ln) {
             const int r = ln & 31;
             const int c = r * 64 + (((2 * ks + (ln >> 5)) ^ swz_key(r)) << 4);
             const int8_t* pa = &sm.q[st][0][0] + wm * 32 * WX * 64 + c;
@@ -508,6 +516,44 @@ mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
                     fa[qa][x], fb[rb][y], acc[qa][rb][x][y], 0, 0, 0);
             });
         };
+        // Single items of the above, for explicit interleaving: MFMA i, slab
+        // request i (queries first), fragment read i (A blocks first).
+        auto mfma_one = [&](auto I, const v4i (&fa)[QL][WX], const v4i (&fb)[RL][WY]) {
+            constexpr int i = decltype(I)::value;
+            constexpr int qa = i / (RL * WX * WY), rb = (i / (WX * WY)) % RL;
+            constexpr int x = (i / WY) % WX, y = i % WY;
+            acc[qa][rb][x][y] = __builtin_amdgcn_mfma_i32_32x32x32_i8(
+                fa[qa][x], fb[rb][y], acc[qa][rb][x][y], 0, 0, 0);
+        };
+        auto request_one = [&](auto I, int kk, int buf, int ln) {
+            constexpr int i = decltype(I)::value;
+            const int64_t k0 = (int64_t)kk * KS;
+            if constexpr (i < QL * QJ) {
+                constexpr int l = i / QJ, t = i % QJ;
+                const int8_t* src;
+                if constexpr (IND) src = gq[l][t] + k0;
+                else src = qsrc[l] + (int64_t)m0 * ld + k0 + tile_off(ln, t, QJ, m0, nq);
+                load_lds16(src, &sm.q[buf][l][1024 * (wave * QJ + t)]);
+            } else {
+                constexpr int l = (i - QL * QJ) / RJ, t = (i - QL * QJ) % RJ;
+                load_lds16(rsrc[l] + (int64_t)n0 * ld + k0 + tile_off(ln, t, RJ, n0, nr),
+                           &sm.r[buf][l][1024 * (wave * RJ + t)]);
+            }
+        };
+        auto frag_one = [&](auto I, v4i (&fa)[QL][WX], v4i (&fb)[RL][WY], int st, int ks, int ln) {
+            constexpr int i = decltype(I)::value;
+            const int r = ln & 31;
+            const int c = r * 64 + (((2 * ks + (ln >> 5)) ^ swz_key(r)) << 4);
+            if constexpr (i < QL * WX) {
+                constexpr int l = i / WX, x = i % WX;
+                fa[l][x] = *reinterpret_cast<const v4i*>(&sm.q[st][0][0] + wm * 32 * WX * 64 + c +
+                                                         l * MT * KS + x * 32 * 64);
+            } else {
+                constexpr int l = (i - QL * WX) / WY, y = (i - QL * WX) % WY;
+                fb[l][y] = *reinterpret_cast<const v4i*>(&sm.r[st][0][0] + wn * 32 * WY * 64 + c +
+                                                         l * NT * KS + y * 32 * 64);
+            }
+        };
 
         // Pipeline.  Slab kk + ST - 1 is requested while slab kk is
         // multiplied; the barrier sits between the two K-steps of a slab, so
@@ -520,33 +566,62 @@ mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
         // in LDS and no wave still reads the stage that the next request
         // overwrites.  (ld >= 128, so nk >= 2 >= ST - 1.)
         static_assert(KS / 32 == 2, "two K-steps per slab");
+        // ln: the lane index, recomputed at the end of each iteration (an
+        // opaque asm is a scheduling boundary, so none sits inside a half)
+        int ln = opaque_lane();
 #pragma unroll
-        for (int s = 0; s < ST - 1; ++s) issue_slab(s, s);
+        for (int s = 0; s < ST - 1; ++s) issue_slab(s, s, ln);
         wait_vmcnt<(ST - 2) * LPS>();
         lds_barrier();
         v4i fa0[QL][WX], fb0[RL][WY], fa1[QL][WX], fb1[RL][WY];
-        load_frags(fa0, fb0, 0, 0);
+        load_frags(fa0, fb0, 0, 0, ln);
         int cur = 0, nxt = ST - 1;  // stage of slab kk, stage for slab kk + ST - 1
+        // Issue order inside each half: the slab requests and fragment reads
+        // are spread over the MFMA stream in program order (each group
+        // fenced by sched_barrier), so that with one wave per SIMD the
+        // matrix pipe is not left idle while they issue.
+        constexpr int NMF = QL * RL * WX * WY;  // MFMAs per K-step
+        constexpr int NFR = QL * WX + RL * WY;  // fragment reads per K-step
+        static_assert(NMF == 16 && NFR == 8 && LPS == 8, "interleave pattern");
         for (int kk = 0; kk + 1 < nk; ++kk) {
-            const bool more = kk + ST - 1 < nk;
-            if (more) issue_slab(kk + ST - 1, nxt);
-            load_frags(fa1, fb1, cur, 1);
-            mfma_all(fa0, fb0);
-            __builtin_amdgcn_sched_barrier(0);
-            if (more) wait_vmcnt_lgkm0<(ST - 2) * LPS>();  // slab kk+1 landed
-            else wait_vmcnt_lgkm0<0>();
+            // past the last slab the request repeats slab nk - 1 into the
+            // stage of slab kk - 1, which no later read uses: the loop body
+            // stays branch-free
+            const int kreq = kk + ST - 1 < nk ? kk + ST - 1 : nk - 1;
+            // fragment reads in the first half of the stream: they have
+            // landed by the wait before the barrier
+            static_for<8>([&](auto I) {
+                constexpr int i = decltype(I)::value;
+                mfma_one(std::integral_constant<int, 2 * i>{}, fa0, fb0);
+                request_one(I, kreq, nxt, ln);
+                if constexpr (i < 4) {
+                    frag_one(std::integral_constant<int, 2 * i>{}, fa1, fb1, cur, 1, ln);
+                    frag_one(std::integral_constant<int, 2 * i + 1>{}, fa1, fb1, cur, 1, ln);
+                }
+                mfma_one(std::integral_constant<int, 2 * i + 1>{}, fa0, fb0);
+                __builtin_amdgcn_sched_barrier(0);
+            });
+            wait_vmcnt_lgkm0<(ST - 2) * LPS>();  // slab kk+1 landed
             lds_barrier();
             cur = cur + 1 == ST ? 0 : cur + 1;
             nxt = nxt + 1 == ST ? 0 : nxt + 1;
-            load_frags(fa0, fb0, cur, 0);
-            mfma_all(fa1, fb1);
-            __builtin_amdgcn_sched_barrier(0);
+            static_for<8>([&](auto I) {
+                constexpr int i = decltype(I)::value;
+                frag_one(I, fa0, fb0, cur, 0, ln);
+                mfma_one(std::integral_constant<int, 2 * i>{}, fa1, fb1);
+                mfma_one(std::integral_constant<int, 2 * i + 1>{}, fa1, fb1);
+                __builtin_amdgcn_sched_barrier(0);
+            });
+            ln = opaque_lane();
         }
         // last slab: already complete in LDS
-        load_frags(fa1, fb1, cur, 1);
+        load_frags(fa1, fb1, cur, 1, ln);
         mfma_all(fa0, fb0);
         mfma_all(fa1, fb1);
-        __syncthreads();  // stage 0 is reused by the epilogue and the next tile
+        // the repeated requests land before the stages turn into the
+        // epilogue's buffers (and the next tile's stages)
+        wait_vmcnt<0>();
+        __syncthreads();
 
         // The epilogue derives its addresses from an opaque copy of the lane
         // index, so that none of them is computed ahead and kept live
@@ -570,6 +645,8 @@ mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
         // offered together and merged (topk_pass_end); pairs whose offer
         // found the buffer full are offered again in the next pass.
         float* stage = reinterpret_cast<float*>(&sm.q[0][0][0]) + wave * 1024 + lane_e;
+        constexpr int CAP = MfmaSmem<QL, RL, KP, ST>::kCap;
+        uint64_t* cbuf = reinterpret_cast<uint64_t*>(&sm.r[0][0][0]);
         static_for<WX>([&](auto X) {
             constexpr int x = decltype(X)::value;
             const int mb = wm * 32 * WX + x * 32 + 4 * (lane_e >> 5);
@@ -622,19 +699,20 @@ mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
                             const int e = __builtin_ctz(hm);
                             hm &= hm - 1;
                             const int m = mb + 8 * (e >> 2) + (e & 3);
-                            if (m0 + m < nq && tl.offer(m, make_key(stage[64 * e], n), pass))
+                            if (m0 + m < nq &&
+                                tl.template offer<CAP>(m, make_key(stage[64 * e], n), pass, cbuf))
                                 ov |= 1u << e;
                         } while (hm);
                         dropped[y] = ov;
                     }
                 });
-                if (!topk_pass_end(tl, tid, pass)) break;
+                if (!topk_pass_end<CAP>(tl, cbuf, tid, pass)) break;
             }
         });
     }
     for (int i = tid; i < MT * KP; i += NTH) {
-        const int m = i / KP, j = i % KP;
-        if (m0 + m < nq) part[((int64_t)split * nq + m0 + m) * KP + j] = tl.top[m][j];
+        const int j = i / MT, m = i % MT;
+        if (m0 + m < nq) part[((int64_t)split * nq + m0 + m) * KP + j] = tl.top[j][m];
     }
 }
 
@@ -910,17 +988,23 @@ inline int choose_splits(hvk_ctx* ctx, int q_tiles, int r_tiles, int wg_per_cu) 
 }
 
 // Reference slices and XCD groups of an MFMA launch (one workgroup per CU).
-// With many query tiles, 8 slices x (CUs per XCD / 8) query tiles per XCD
-// group; with few, as many slices as fill the device.  want_splits > 0
-// forces the slice count.
+// Automatic choice: 4 slices, or as many as fill the device when there are
+// few query tiles (measured at 64k-1M reads x 100k references, D = 10240:
+// 4 slices beat 1, 2, 8 and 16 by 2-15%; every slice rebuilds its candidate
+// lists from an empty start, while one slice leaves the last round of
+// workgroups part-empty).  want_splits > 0 (the exact second pass) or the
+// context's tuning value forces the slice count.
 struct MfmaPlan {
     TileMap map;
     int blocks;
 };
 inline MfmaPlan plan_mfma(hvk_ctx* ctx, int q_tiles, int r_tiles, int want_splits) {
     const int per_xcd = ctx->n_cu / kXcds > 0 ? ctx->n_cu / kXcds : 1;
-    int s = want_splits;
-    if (s <= 0) s = (int64_t)q_tiles * 8 >= 2LL * ctx->n_cu ? 8 : (ctx->n_cu + q_tiles - 1) / q_tiles;
+    int s = want_splits > 0 ? want_splits : ctx->tune_splits;
+    if (s <= 0) {
+        s = (ctx->n_cu + q_tiles - 1) / q_tiles;
+        if (s < 4) s = 4;
+    }
     if (s > r_tiles) s = r_tiles;
     if (s > 64) s = 64;
     if (s < 1) s = 1;

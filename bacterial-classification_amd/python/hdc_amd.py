@@ -28,8 +28,10 @@ EXPORTS = [
     "hvk_ctx_info", "hvk_base_hv", "hvk_pack_words", "hvk_pack", "hvk_encode",
     "hvk_search_plain", "hvk_search_mfma", "hvk_gen_size", "hvk_gen_fill",
     "hvk_ctx_set_timing", "hvk_ctx_last_timing", "hvk_encode_ex", "hvk_encode_windows",
-    "hvk_search_mfma_bounded", "hvk_prepare_counts",
+    "hvk_search_mfma_bounded", "hvk_prepare_counts", "hvk_ctx_set_tuning",
 ]
+
+TUNE_SPLITS = 1
 
 
 class HvkError(RuntimeError):
@@ -105,6 +107,7 @@ def lib():
                                           i64, i32, vp, vp, vp, vp, vp]
     L.hvk_prepare_counts.argtypes = [vp, vp, i32, ctypes.POINTER(EncodeOut), vp]
     L.hvk_ctx_set_timing.argtypes = [vp, i32]
+    L.hvk_ctx_set_tuning.argtypes = [vp, i32, i32]
     L.hvk_ctx_last_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
     L.hvk_gen_size.argtypes = [ctypes.POINTER(GenSpec), ctypes.POINTER(i64), ctypes.POINTER(i64)]
     L.hvk_gen_fill.argtypes = [ctypes.POINTER(GenSpec), vp, vp, vp, vp, vp]
@@ -215,6 +218,9 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def set_tuning(self, param: int, value: int):
+        check(lib().hvk_ctx_set_tuning(self.h, int(param), int(value)), "hvk_ctx_set_tuning")
 
     def set_timing(self, enable: bool):
         check(lib().hvk_ctx_set_timing(self.h, int(enable)), "hvk_ctx_set_timing")

@@ -70,6 +70,13 @@ int hvk_base_hv(const hvk_ctx* ctx, char symbol, uint64_t* host_out);
 int hvk_ctx_set_timing(hvk_ctx* ctx, int enable);
 int hvk_ctx_last_timing(hvk_ctx* ctx, float* encode_ms, float* search_ms);
 
+/* Tuning of the MFMA search launch (no counterpart in the reference; used by
+ * the tile autotuner).  HVK_TUNE_SPLITS: number of reference slices per
+ * query tile, 0 = automatic (the default), else 1..64.  Results do not
+ * depend on it. */
+#define HVK_TUNE_SPLITS 1
+int hvk_ctx_set_tuning(hvk_ctx* ctx, int param, int value);
+
 /* ---- packing (host) -----------------------------------------------------
  * Sequences are concatenated: sequence i is seqs[offsets[i] .. offsets[i+1]).
  * packed: 2-bit codes A0 C1 G2 T3, base p at bits 2*(p%16) of word p/16.

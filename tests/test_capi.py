@@ -55,6 +55,7 @@ def test_null_context_is_an_argument_error():
                              None, None, None) == H.HVK_ERR_ARG
     assert L.hvk_ctx_destroy(None) == H.HVK_ERR_ARG
     assert L.hvk_ctx_set_timing(None, 1) == H.HVK_ERR_ARG
+    assert L.hvk_ctx_set_tuning(None, H.TUNE_SPLITS, 8) == H.HVK_ERR_ARG
 
 
 def test_pack_layout():
