@@ -8,10 +8,12 @@ namespace hvk {
 __global__ void inv_norm_kernel(const int64_t* __restrict__ qn, int nq, const int64_t* __restrict__ rn,
                                 int nr, float* __restrict__ iq, float* __restrict__ ir,
                                 int32_t* __restrict__ flag_count, BoundRows bnd,
-                                float2* __restrict__ bq, float2* __restrict__ br) {
+                                float2* __restrict__ bq, float2* __restrict__ br,
+                                uint64_t* __restrict__ qthr) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) { flag_count[0] = 0; flag_count[32] = 0; }
     if (i < nq) {
+        qthr[i] = 0;
         const int64_t v = qn[i];
         const float x = v == 0 ? 0.0f : __fdiv_rn(1.0f, __fsqrt_rn((float)v));
         iq[i] = x;
@@ -35,7 +37,7 @@ void launch_inv_norm(const int64_t* qn, int nq, const int64_t* rn, int nr, const
     const int nmax = nq > nr ? nq : nr;
     hipLaunchKernelGGL(inv_norm_kernel, dim3((nmax + 255) / 256), dim3(256), 0, st, qn, nq, rn, nr,
                        b.iq, b.ir, b.flag_count, bnd, bounds ? b.bq : nullptr,
-                       bounds ? b.br : nullptr);
+                       bounds ? b.br : nullptr, b.qthr);
 }
 
 template <typename Rows>

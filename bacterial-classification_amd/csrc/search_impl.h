@@ -385,7 +385,12 @@ mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
             const int8_t* __restrict__ rhi, const float* __restrict__ ir, int nr, int64_t ld,
             TileMap map, uint64_t* __restrict__ part, const float2* __restrict__ bq,
             const float2* __restrict__ br, const int32_t* __restrict__ q_rows,
-            const int32_t* __restrict__ q_count) {
+            const int32_t* __restrict__ q_count, uint64_t* __restrict__ qthr) {
+    // qthr (nullable, zeroed by the caller): per query, the largest
+    // threshold key any slice has reached.  Every slice publishes its own
+    // after each tile and adopts the largest: a slice's KP-th best key is a
+    // lower bound on the KP-th best of the merged lists, so the keys it
+    // rejects could not have entered the merged top KP.
     using T = MfmaTile<QL, RL>;
     constexpr int WX = T::WX, WY = T::WY, MT = T::MT, NT = T::NT, KS = kMfmaKS;
     constexpr int ST = mfma_stages<QL, RL, KP>();
@@ -644,6 +649,60 @@ mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
         // and offers the survivors one by one.  The blocks of one x are
         // offered together and merged (topk_pass_end); pairs whose offer
         // found the buffer full are offered again in the next pass.
+        // FP32 score (upper bound with BND) of pair e of block (x, y)
+        auto pair_score = [&](auto X, auto Y, auto E, float iqh, float2 bqh) -> float {
+            constexpr int x = decltype(X)::value, y = decltype(Y)::value, e = decltype(E)::value;
+            float v = 0.0f;  // sum_{qa,rb} acc * 256^(qa+rb) in FP32
+            static_for<QL * RL>([&](auto J) {
+                constexpr int qa = decltype(J)::value / RL, rb = decltype(J)::value % RL;
+                constexpr float scale = (float)(1u << (8 * (qa + rb)));
+                v = __fmaf_rn((float)acc[qa][rb][x][y][e], scale, v);
+            });
+            float sc = __fmul_rn(__fmul_rn(v, iqh), irv[y]);
+            if constexpr (BND) sc = __fmaf_rn(bqh.y, brv[y].y, __fmaf_rn(bqh.x, brv[y].x, sc));
+            return sc;
+        };
+        // First tile of the slice: the lists are empty, and without a
+        // threshold nearly every pair would be offered.  Raise each query's
+        // threshold to a floor first: the lanes of a half-wave are split into
+        // KP groups (32 / KP lanes, WY references each); the minimum over
+        // the groups of each group's best score is reached by KP distinct
+        // references of the tile, so it does not exceed the query's KP-th
+        // best key, and keys below it could not enter the top KP.
+        if (n0 == n_begin) {
+            uint32_t* floor_o = reinterpret_cast<uint32_t*>(&sm.q[1 % ST][0][0]);
+            for (int i = tid; i < MT; i += NTH) floor_o[i] = 0;
+            __syncthreads();
+            static_for<WX>([&](auto X) {
+                constexpr int x = decltype(X)::value;
+                const int mb = wm * 32 * WX + x * 32 + 4 * (lane_e >> 5);
+                static_for<16>([&](auto E) {
+                    constexpr int e = decltype(E)::value;
+                    const int row = mb + 8 * (e >> 2) + (e & 3);
+                    const float iqh = sm.iq[row];
+                    const float2 bqh = sm.bq[row];
+                    float v = -INFINITY;
+                    static_for<WY>([&](auto Y) {
+                        constexpr int y = decltype(Y)::value;
+                        const int n = n0 + wn * 32 * WY + y * 32 + (lane_e & 31);
+                        const float sc = pair_score(X, Y, E, iqh, bqh);
+                        if (n < nr) v = fmaxf(v, sc);
+                    });
+                    for (int o = 1; o < 32 / KP; o <<= 1) v = fmaxf(v, __shfl_xor(v, o));
+                    for (int o = 1; o < 32; o <<= 1) v = fminf(v, __shfl_xor(v, o));
+                    if ((lane_e & 31) == 0) atomicMax(&floor_o[row], ord_f32(v));
+                });
+            });
+            __syncthreads();
+            for (int i = tid; i < MT; i += NTH) {
+                const uint64_t key = (uint64_t)floor_o[i] << 32;
+                if (key > tl.thr[i]) {
+                    tl.thr[i] = key;
+                    tl.thr_f[i] = f32_of_ord(floor_o[i]);
+                }
+            }
+            __syncthreads();
+        }
         float* stage = reinterpret_cast<float*>(&sm.q[0][0][0]) + wave * 1024 + lane_e;
         constexpr int CAP = MfmaSmem<QL, RL, KP, ST>::kCap;
         uint64_t* cbuf = reinterpret_cast<uint64_t*>(&sm.r[0][0][0]);
@@ -674,15 +733,8 @@ mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
                         }
                         static_for<4>([&](auto H) {
                             constexpr int h = 3 - decltype(H)::value, e = 4 * gg + h;
-                            float v = 0.0f;  // sum_{qa,rb} acc * 256^(qa+rb) in FP32
-                            static_for<QL * RL>([&](auto J) {
-                                constexpr int qa = decltype(J)::value / RL, rb = decltype(J)::value % RL;
-                                constexpr float scale = (float)(1u << (8 * (qa + rb)));
-                                v = __fmaf_rn((float)acc[qa][rb][x][y][e], scale, v);
-                            });
-                            sv[e] = __fmul_rn(__fmul_rn(v, iqv[h]), irv[y]);
-                            if constexpr (BND)
-                                sv[e] = __fmaf_rn(bqv[h].y, brv[y].y, __fmaf_rn(bqv[h].x, brv[y].x, sv[e]));
+                            sv[e] = pair_score(X, Y, std::integral_constant<int, e>{}, iqv[h],
+                                               BND ? bqv[h] : make_float2(0.0f, 0.0f));
                             hm = (hm << 1) | (sv[e] >= thv[h] ? 1u : 0u);
                         });
                     });
@@ -709,6 +761,15 @@ mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
                 if (!topk_pass_end<CAP>(tl, cbuf, tid, pass)) break;
             }
         });
+        if (qthr && tid < MT && m0 + tid < nq) {
+            const uint64_t mine = tl.thr[tid];
+            const uint64_t g = atomicMax((unsigned long long*)&qthr[m0 + tid],
+                                         (unsigned long long)mine);
+            if (g > mine) {
+                tl.thr[tid] = g;
+                tl.thr_f[tid] = f32_of_ord((uint32_t)(g >> 32));
+            }
+        }
     }
     for (int i = tid; i < MT * KP; i += NTH) {
         const int j = i / MT, m = i % MT;
@@ -948,6 +1009,7 @@ struct SearchBufs {
     uint64_t* part2;      // second (exact) pass
     int32_t* flag_list2;  // queued by the second finalize
     int32_t* flag_count2;
+    uint64_t* qthr;       // per query: the largest threshold key any slice has reached
 };
 
 constexpr int kPass2Cap = 16384;  // queries the exact second pass takes at once
@@ -955,7 +1017,9 @@ constexpr int kPass2Splits = 16;
 constexpr int kPass2KP = 32;
 
 inline int prepare(hvk_ctx* ctx, int nq, int nr, int n_splits, int kp, SearchBufs& b) {
-    const size_t need_n = sizeof(float) * (3 * ((size_t)nq + nr) + 512);
+    const size_t nqp = ((size_t)nq + 63) / 64 * 64, nrp = ((size_t)nr + 63) / 64 * 64;
+    const size_t need_n = sizeof(float) * (nqp + nrp) + sizeof(float2) * (nqp + nrp) +
+                          sizeof(uint64_t) * nqp;
     int st = ctx->ws_norms.ensure(need_n);
     if (st) return st;
     st = ctx->ws_search.ensure(sizeof(uint64_t) * (size_t)n_splits * nq * kp);
@@ -965,9 +1029,10 @@ inline int prepare(hvk_ctx* ctx, int nq, int nr, int n_splits, int kp, SearchBuf
     st = ctx->ws_search2.ensure(sizeof(uint64_t) * (size_t)kPass2Splits * kPass2Cap * kPass2KP);
     if (st) return st;
     b.iq = (float*)ctx->ws_norms.ptr;
-    b.ir = b.iq + ((nq + 63) / 64) * 64;
-    b.bq = reinterpret_cast<float2*>(b.ir + ((nr + 63) / 64) * 64);
-    b.br = b.bq + ((nq + 63) / 64) * 64;
+    b.ir = b.iq + nqp;
+    b.bq = reinterpret_cast<float2*>(b.ir + nrp);
+    b.br = b.bq + nqp;
+    b.qthr = reinterpret_cast<uint64_t*>(b.br + nrp);
     b.part = (uint64_t*)ctx->ws_search.ptr;
     b.flag_count = (int32_t*)ctx->ws_flags.ptr;
     b.flag_count2 = b.flag_count + 32;
@@ -1095,7 +1160,7 @@ int run_mfma(hvk_ctx* ctx, const int8_t* qlo, const int8_t* qhi, const int64_t* 
     timing_mark(ctx, 1, false, st);
     hipLaunchKernelGGL((mfma_kernel<QL, RL, KP>), dim3(pl.blocks), dim3(kMfmaThreads), 0, st,
                        qlo, qhi, b.iq, nq, rlo, rhi, b.ir, nr, ld, pl.map, b.part, nullptr,
-                       nullptr, nullptr, nullptr);
+                       nullptr, nullptr, nullptr, b.qthr);
     timing_mark(ctx, 1, true, st);
     HVK_HIP(hipGetLastError());
     return finish<RowsLimb, KP>(ctx, RowsLimb{qlo, QL > 1 ? qhi : nullptr, rlo,
@@ -1117,7 +1182,7 @@ int run_mfma_bounded(hvk_ctx* ctx, const hvk_bundles& q, const hvk_bundles& r, i
     timing_mark(ctx, 1, false, st);
     hipLaunchKernelGGL((mfma_kernel<1, 1, KP, true>), dim3(pl.blocks), dim3(kMfmaThreads), 0, st,
                        q.clamp8, nullptr, b.iq, nq, r.clamp8, nullptr, b.ir, nr, ld, pl.map,
-                       b.part, b.bq, b.br, nullptr, nullptr);
+                       b.part, b.bq, b.br, nullptr, nullptr, b.qthr);
     timing_mark(ctx, 1, true, st);
     HVK_HIP(hipGetLastError());
     const RowsLimb rows{q.lo, q.hi, r.lo, r.hi, ld};

@@ -19,7 +19,7 @@ void launch_pass2(hvk_ctx* ctx, const hvk_bundles& q, const hvk_bundles& r, int6
                        dim3(kMfmaThreads), 0, st,
                        q.lo, QL > 1 ? q.hi : nullptr, b.iq, cap, r.lo, RL > 1 ? r.hi : nullptr,
                        b.ir, r.n, ld, pl.map, b.part2, nullptr, nullptr, b.flag_list,
-                       b.flag_count);
+                       b.flag_count, nullptr);
 }
 
 int exact_pass2(hvk_ctx* ctx, const hvk_bundles& q, const hvk_bundles& r, int64_t ld, int k,
