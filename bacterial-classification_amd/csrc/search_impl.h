@@ -291,14 +291,19 @@ plain_kernel(const int16_t* __restrict__ q, const float* __restrict__ iq, int nq
 //     query tile is read by n_splits workgroups and each reference tile by
 //     gq workgroups of the same XCD at about the same time, from its L2.
 // ------------------------------------------------------------------------
-constexpr int kMfmaKS = 64, kMfmaThreads = 256, kXcds = 8;
+constexpr int kMfmaKS = 64, kXcds = 8;
 constexpr int kLdsBytes = 160 * 1024;
+// 8 waves, two per SIMD, in a GM x GN = 2 x 4 grid: while one wave of a
+// SIMD waits (slab requests, barrier, fragment reads), the other keeps the
+// matrix pipe busy.  128 accumulator registers per wave.
+constexpr int kMfmaGM = 2, kMfmaGN = 4, kMfmaWaves = kMfmaGM * kMfmaGN;
+constexpr int kMfmaThreads = 64 * kMfmaWaves;
 
 template <int QL, int RL>
-struct MfmaTile {  // wave tile blocks: QL * RL * WX * WY = 16
+struct MfmaTile {  // wave tile blocks: QL * RL * WX * WY = 8
     static constexpr int WX = QL == 1 ? 4 : 2;
-    static constexpr int WY = RL == 1 ? 4 : 2;
-    static constexpr int MT = 64 * WX, NT = 64 * WY;
+    static constexpr int WY = RL == 1 ? 2 : 1;
+    static constexpr int MT = kMfmaGM * 32 * WX, NT = kMfmaGN * 32 * WY;
 };
 
 struct TileMap {
@@ -395,8 +400,9 @@ mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
     constexpr int WX = T::WX, WY = T::WY, MT = T::MT, NT = T::NT, KS = kMfmaKS;
     constexpr int ST = mfma_stages<QL, RL, KP>();
     constexpr int NTH = kMfmaThreads;
-    constexpr int QJ = MT * KS / 1024 / 4;  // wave-loads per wave per Q limb
-    constexpr int RJ = NT * KS / 1024 / 4;  // per R limb
+    constexpr int QJ = MT * KS / 1024 / kMfmaWaves;  // wave-loads per wave per Q limb
+    constexpr int RJ = NT * KS / 1024 / kMfmaWaves;  // per R limb
+    static_assert(QJ >= 1 && RJ >= 1, "every wave requests a share of each operand");
     constexpr int LPS = QL * QJ + RL * RJ;  // wave-loads per wave per slab
     __shared__ __attribute__((aligned(16))) MfmaSmem<QL, RL, KP, ST> sm;
     auto& tl = sm.tl;
@@ -418,7 +424,7 @@ mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
-    const int wm = wave & 1, wn = wave >> 1;
+    const int wm = wave % kMfmaGM, wn = wave / kMfmaGM;
     auto qrow = [&](int m) { return IND ? q_rows[m] : m; };
     tl.init(tid, NTH);
     for (int i = tid; i < MT; i += NTH) {
@@ -587,22 +593,24 @@ mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
         // matrix pipe is not left idle while they issue.
         constexpr int NMF = QL * RL * WX * WY;  // MFMAs per K-step
         constexpr int NFR = QL * WX + RL * WY;  // fragment reads per K-step
-        static_assert(NMF == 16 && NFR == 8 && LPS == 8, "interleave pattern");
+        constexpr int NSL = NMF / 2;            // slots of two MFMAs
+        static_assert(NMF % 2 == 0 && LPS <= NSL && NFR <= 2 * NSL, "interleave pattern");
         for (int kk = 0; kk + 1 < nk; ++kk) {
             // past the last slab the request repeats slab nk - 1 into the
             // stage of slab kk - 1, which no later read uses: the loop body
             // stays branch-free
             const int kreq = kk + ST - 1 < nk ? kk + ST - 1 : nk - 1;
-            // fragment reads in the first half of the stream: they have
-            // landed by the wait before the barrier
-            static_for<8>([&](auto I) {
+            // first half: slab requests spread over the slots, fragment
+            // reads front-loaded (they have landed by the wait before the
+            // barrier)
+            static_for<NSL>([&](auto I) {
                 constexpr int i = decltype(I)::value;
                 mfma_one(std::integral_constant<int, 2 * i>{}, fa0, fb0);
-                request_one(I, kreq, nxt, ln);
-                if constexpr (i < 4) {
+                if constexpr (i < LPS) request_one(I, kreq, nxt, ln);
+                if constexpr (2 * i < NFR)
                     frag_one(std::integral_constant<int, 2 * i>{}, fa1, fb1, cur, 1, ln);
+                if constexpr (2 * i + 1 < NFR)
                     frag_one(std::integral_constant<int, 2 * i + 1>{}, fa1, fb1, cur, 1, ln);
-                }
                 mfma_one(std::integral_constant<int, 2 * i + 1>{}, fa0, fb0);
                 __builtin_amdgcn_sched_barrier(0);
             });
@@ -610,10 +618,13 @@ mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
             lds_barrier();
             cur = cur + 1 == ST ? 0 : cur + 1;
             nxt = nxt + 1 == ST ? 0 : nxt + 1;
-            static_for<8>([&](auto I) {
+            // second half: next slab's first-step fragments spread over it
+            static_for<NSL>([&](auto I) {
                 constexpr int i = decltype(I)::value;
-                frag_one(I, fa0, fb0, cur, 0, ln);
+                if constexpr (i < NFR) frag_one(I, fa0, fb0, cur, 0, ln);
                 mfma_one(std::integral_constant<int, 2 * i>{}, fa1, fb1);
+                if constexpr (i + NSL < NFR)
+                    frag_one(std::integral_constant<int, i + NSL>{}, fa0, fb0, cur, 0, ln);
                 mfma_one(std::integral_constant<int, 2 * i + 1>{}, fa1, fb1);
                 __builtin_amdgcn_sched_barrier(0);
             });
@@ -645,8 +656,8 @@ mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
         // reference column n0 + wn*32*WY + y*32 + lane%32 (the 32x32 MFMA
         // output layout).  A float prefilter against the query's current
         // FP32 threshold rejects almost every pair; a lane with survivors in
-        // a block stages that block's 16 scores in LDS (the idle stage 0)
-        // and offers the survivors one by one.  The blocks of one x are
+        // a block stages that block's 16 scores in LDS (the idle query
+        // stages) and offers the survivors one by one.  The blocks of one x are
         // offered together and merged (topk_pass_end); pairs whose offer
         // found the buffer full are offered again in the next pass.
         // FP32 score (upper bound with BND) of pair e of block (x, y)
@@ -670,7 +681,8 @@ mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
         // references of the tile, so it does not exceed the query's KP-th
         // best key, and keys below it could not enter the top KP.
         if (n0 == n_begin) {
-            uint32_t* floor_o = reinterpret_cast<uint32_t*>(&sm.q[1 % ST][0][0]);
+            // (the reference stages hold the floors; the offers below reuse them)
+            uint32_t* floor_o = reinterpret_cast<uint32_t*>(&sm.r[0][0][0]);
             for (int i = tid; i < MT; i += NTH) floor_o[i] = 0;
             __syncthreads();
             static_for<WX>([&](auto X) {
@@ -703,6 +715,7 @@ mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
             }
             __syncthreads();
         }
+        static_assert(sizeof(sm.q) >= kMfmaWaves * 64 * 16 * sizeof(float), "score stash");
         float* stage = reinterpret_cast<float*>(&sm.q[0][0][0]) + wave * 1024 + lane_e;
         constexpr int CAP = MfmaSmem<QL, RL, KP, ST>::kCap;
         uint64_t* cbuf = reinterpret_cast<uint64_t*>(&sm.r[0][0][0]);
