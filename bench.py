@@ -14,6 +14,7 @@ timed on this host) on a bounded sample, whose top-5 must equal the GPU's.
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -25,6 +26,21 @@ sys.path.insert(0, os.path.join(ROOT, "bacterial-classification_amd", "python"))
 DIM, KMER, TOPK = 10240, 12, 5
 MFMA_INT8_PEAK_TOPS = 5000.0  # MI355X dense int8 MFMA, TOPS
 HBM_PEAK_GBS = 8000.0
+PMC_PROFILE = os.path.join(ROOT, "profiles", "mfma_kernel_config1_pmc.md")
+
+
+def pmc_traffic(kernel, reads, refs):
+    """HBM bytes per launch of `kernel` on this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (copy-calibrated); None when the
+    profile does not cover this kernel and size."""
+    try:
+        text = open(PMC_PROFILE).read()
+    except OSError:
+        return None
+    for m in re.finditer(r"traffic_per_launch: kernel=(\S+) reads=(\d+) refs=(\d+) bytes=(\d+)", text):
+        if m.group(1) == kernel and int(m.group(2)) == reads and int(m.group(3)) == refs:
+            return float(m.group(4))
+    return None
 
 
 def parse():
@@ -230,7 +246,10 @@ def main():
                    "read_limbs": 2 if q_two else 1, "ref_limbs": 2 if db_two_limbs else 1},
         "roofline": {"bound": "mfma", "achieved": achieved_tops, "peak": MFMA_INT8_PEAK_TOPS,
                      "unit": "TFLOP/s", "frac": achieved_tops / MFMA_INT8_PEAK_TOPS,
-                     "traffic": None, "kernel": "mfma_kernel<1,1,16,true>",
+                     "traffic": pmc_traffic("mfma_kernel<1,1,16,true>", nq, n_refs),
+                     "traffic_source": "rocprofv3 --pmc FETCH_SIZE x2.0 + WRITE_SIZE x1.0 "
+                                       "(copy-calibrated), profiles/mfma_kernel_config1_pmc.md",
+                     "kernel": "mfma_kernel<1,1,16,true>",
                      "kernel_ms": float(np.mean(srch_ms))},
         "encode": {"kernel_ms": float(np.mean(enc_ms)), "achieved_gbs": enc_gbs,
                    "frac_hbm": enc_gbs / HBM_PEAK_GBS},
