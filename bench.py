@@ -55,6 +55,11 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-two-limb", action="store_true")
+    ap.add_argument("--workload", choices=["classify", "precision"], default="classify",
+                    help="classify: the reads/s line (1M reads x 100k refs, top-5); precision: "
+                         "per-result error vs FP64 at 500k refs, 2M reads of 250 nt, top-10")
+    ap.add_argument("--precision-reads", type=int, default=2_000_000)
+    ap.add_argument("--precision-ancestors", type=int, default=5000)
     ap.add_argument("--copy-calib-mb", type=int, default=0,
                     help="first run a device-to-device copy of this many MiB (a kernel of known "
                          "traffic, for scaling HBM byte counters)")
@@ -71,6 +76,9 @@ def main():
     import hdc_amd as H
 
     torch.cuda.set_device(0)
+    if a.workload == "precision":
+        run_precision(a)
+        return
     if a.copy_calib_mb > 0:
         nbytes = a.copy_calib_mb << 20
         src = torch.ones(nbytes, dtype=torch.uint8, device="cuda")
@@ -259,6 +267,119 @@ def main():
         "n_rescored": n_res,
         "db_encode_s": db_encode_s,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def run_precision(a):
+    """Precision form: 500k references (5,000 ancestors x 100), 2M reads of 250 nt, top-10.
+
+    Both MFMA forms run on the same encoded reads: the bounded one-limb form (clamped int8
+    operands, FP32 keys carrying an exact upper bound) and the two-limb form (counts split into
+    hi/lo int8 limbs, exact int32 partial products).  Each returned score is float(c) of the
+    FP64 c recomputed from the exact int64 dot; the report gives, for every one of the
+    2M x 10 results, its relative error against that FP64 value, and checks a sample of reads
+    against the oracle's FP64 brute force (indices identical, FP64 values equal)."""
+    import torch
+    import hdc_amd as H
+    K10 = 10
+    t0 = time.time()
+    spec = H.make_spec(seed=0, n_ancestors=a.precision_ancestors, desc_per_ancestor=100,
+                       anc_len_min=1300, anc_len_max=1600, div_min=0.02, div_max=0.08,
+                       indel_rate=0.01, n_reads=a.precision_reads, read_len_min=250,
+                       read_len_max=250, err_min=0.005, err_max=0.02)
+    w = H.generate(spec)
+    n_refs, nq = len(w.ref_offsets) - 1, a.precision_reads
+    print(f"[precision] generated {n_refs} refs, {nq} reads in {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    ctx = H.Context(DIM, KMER)
+    ctx.set_timing(True)
+    db = H.encode(ctx, H.DeviceSeqs(w.refs, w.ref_offsets), counts=True, limbs=True, hi=True,
+                  clamp=True)
+    read_seqs = H.DeviceSeqs(w.reads, w.read_offsets)
+    reads = H.encode(ctx, read_seqs, counts=False, limbs=True, hi=True, clamp=True)
+    torch.cuda.synchronize()
+    print(f"[precision] encoded (max|count| refs {int(db.maxabs.max())}, reads "
+          f"{int(reads.maxabs.max())})", file=sys.stderr, flush=True)
+
+    res = {}
+    for form in ("bounded", "two_limb"):
+        out = H._topk_bufs(nq, K10)
+        run = (lambda: H.search_mfma_bounded(ctx, reads, db, K10, out=out)) if form == "bounded" \
+            else (lambda: H.search_mfma(ctx, reads, db, K10, out=out))
+        run()  # warm-up
+        torch.cuda.synchronize()
+        ts = time.time()
+        run()
+        torch.cuda.synchronize()
+        sec = time.time() - ts
+        _, kms = ctx.last_timing()
+        res[form] = (out, sec, kms)
+        print(f"[precision] {form}: {nq / sec:.0f} reads/s (search kernel {kms:.1f} ms), "
+              f"re-scored {int(out.n_rescored.item())}", file=sys.stderr, flush=True)
+
+    out, sec, kms = res["bounded"]
+    sc = out.score.double()
+    c64 = out.cos64
+    ok = out.idx >= 0
+    rel = ((sc - c64).abs() / c64.abs().clamp_min(1e-300))[ok]
+    absd = (sc - c64).abs()[ok]
+    same_forms = bool(torch.equal(out.idx, res["two_limb"][0].idx))
+    # margin of the k-th result over the (k+1)-th: how near the ties are
+    gap = (c64[:, K10 - 2] - c64[:, K10 - 1])[ok[:, K10 - 1]]
+
+    cpu = None
+    if not a.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py as orc
+        ncores = min(16, os.cpu_count() or 1)
+        db_host = np.ascontiguousarray(db.counts.cpu().numpy()[:, :DIM])
+        dbn = db.sqnorm.cpu().numpy()
+        O = orc.oracle()
+
+        def cpu_run(n):
+            seqs = w.reads[: w.read_offsets[n]]
+            offs = np.ascontiguousarray(w.read_offsets[: n + 1])
+            oi = np.zeros((n, K10), np.int32)
+            oc = np.zeros((n, K10), np.float64)
+            t = O.orc_classify_reads(DIM, KMER, seqs, offs.ctypes.data, n, db_host.ctypes.data,
+                                     dbn.ctypes.data, n_refs, K10, oi.ctypes.data, oc.ctypes.data,
+                                     ncores)
+            return t, oi, oc
+
+        t1, _, _ = cpu_run(ncores)
+        n_cpu = int(max(ncores, min(nq, (a.cpu_seconds / (t1 / ncores)) // ncores * ncores)))
+        tc, oi, oc = cpu_run(n_cpu)
+        gi = out.idx[:n_cpu].cpu().numpy()
+        gc = c64[:n_cpu].cpu().numpy()
+        cpu = {"reads": n_cpu, "cores": ncores, "reads_per_s": n_cpu / tc,
+               "top10_equal": bool(np.array_equal(oi, gi)),
+               "cos64_max_abs_diff": float(np.abs(oc - gc).max()),
+               "score_max_rel_err": float((np.abs(out.score[:n_cpu].cpu().numpy().astype(np.float64)
+                                                  - oc) / np.abs(oc)).max())}
+        print(f"[precision] oracle sample: {n_cpu} reads, top-10 equal {cpu['top10_equal']}",
+              file=sys.stderr, flush=True)
+
+    line = {
+        "metric": "per-result error vs FP64 (precision form, D=10240, 500k refs, 2M reads, top-10)",
+        "value": float(rel.max().item()), "unit": "max relative error of float(c) vs FP64 c",
+        "n_gpus": 1, "higher_is_better": False, "dtype": "int8",
+        "data": "synthetic (deterministic generator, seed 0)",
+        "config": {"workload": f"config 2: D={DIM}, k={KMER}, {n_refs} refs "
+                               f"({a.precision_ancestors} ancestors x 100), {nq} reads of 250 nt, "
+                               f"top-{K10}", "reads": nq, "refs": n_refs, "topk": K10},
+        "results": int(ok.sum().item()),
+        "rel_err": {"max": float(rel.max().item()), "mean": float(rel.mean().item()),
+                    "bound": 2.0 ** -24},
+        "abs_err_max": float(absd.max().item()),
+        "kth_gap_min": float(gap.min().item()) if gap.numel() else None,
+        "kth_gap_below_2e-6": int((gap < 2e-6).sum().item()),
+        "bounded": {"reads_per_s": nq / sec, "kernel_ms": kms,
+                    "n_rescored": int(out.n_rescored.item())},
+        "two_limb": {"reads_per_s": nq / res["two_limb"][1], "kernel_ms": res["two_limb"][2],
+                     "n_rescored": int(res["two_limb"][0].n_rescored.item()),
+                     "top10_equal_bounded": same_forms},
+        "oracle_sample": cpu,
     }
     print(json.dumps(line), flush=True)
 
