@@ -1,0 +1,648 @@
+// descend.hip -- hierarchical descent on the GPU: identify's rank-by-rank
+// beam (targets/identify.cpp:89-202) with one integer prototype bundle per
+// category, scored by the reference's cosine (src/HV.cpp:219-242) from exact
+// int64 dots.
+//
+// One level is six launches on the caller's stream, with no host round trip:
+//   1. bucket count: one key per live (read, beam entry) = its category
+//   2. bucket scan (one workgroup)
+//   3. scatter: (read, entry) pairs grouped by category
+//   4. score: every pair against the candidate list of its category.  The
+//      lanes of a wave that share a category score it together: the
+//      category is made wave-uniform, so its children's limb rows are read
+//      through the scalar cache once per wave, while each lane streams its
+//      own read row.  v_dot2_i32_i16 multiplies int16 counts by the int16
+//      limbs (p = 65536 hi + lo) into int32 partial sums, widened to int64
+//      every E elements, E chosen so no partial can overflow.
+//   5. select: per read, gather its candidates in identify's order (beam
+//      order, then each entry's candidate list), sort them exactly as
+//      std::sort does (lsort.h), then exit or keep the beam.
+// A final launch writes the surviving reads' best paths.
+#include <algorithm>
+#include <vector>
+
+#include "hvk_internal.h"
+#include "lsort.h"
+
+namespace hvk {
+namespace {
+
+constexpr int kDescG = 8;  // candidates scored per pass over a read row
+constexpr int kDescThreads = 256;
+constexpr int kMaxBeam = 16;
+constexpr int kMaxLevels = 32;
+
+typedef short short2_t __attribute__((ext_vector_type(2)));
+
+// Beam state of one level, structure of arrays over reads.
+struct Beams {
+    int32_t* nb;    // [n] live entries (0: the read has exited)
+    int32_t* cat;   // [n][B] last category of each entry (-1: the root)
+    int32_t* path;  // [n][B][L]
+    float* pscore;  // [n][B][L]
+};
+
+struct TreeDev {
+    const int16_t* lo;
+    const int16_t* hi;
+    const int64_t* sqnorm;
+    int64_t ld;
+    int n_cat;
+    const int32_t* child_off;  // [n_cat + 1]
+    const int32_t* child_idx;
+    const int32_t* roots;
+    int n_roots;
+};
+
+// src/HV.cpp:236-241 on the exact integer dot and norms
+__device__ __forceinline__ float ref_score(int64_t dot, int64_t nq, int64_t np) {
+    if (nq == 0 || np == 0) return -1.0f;
+    return (float)(2.0 * ((double)dot / (sqrt((double)nq) * sqrt((double)np)) - 0.5));
+}
+
+// candidate list that follows category c (c < 0: the root)
+__device__ __forceinline__ void span_of(const TreeDev& t, int c, const int32_t*& list, int& n) {
+    if (c < 0) {
+        list = t.roots;
+        n = t.n_roots;
+    } else {
+        const int a = t.child_off[c];
+        list = t.child_idx + a;
+        n = t.child_off[c + 1] - a;
+    }
+}
+
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+// ctr[key] += 1 for every lane with act set, one atomic per distinct key in
+// the wave; returns each such lane's slot (the counter value it took).  All
+// lanes of the wave must call it.
+__device__ __forceinline__ int wave_agg_add(int32_t* ctr, int key, bool act) {
+    int slot = 0;
+    const int lane = lane_id();
+    uint64_t todo = __ballot(act);
+    while (todo) {
+        const int leader = __ffsll((unsigned long long)todo) - 1;
+        const int k = __shfl(key, leader);
+        const uint64_t m = __ballot(act && key == k) & todo;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&ctr[k], __popcll(m));
+        base = __shfl(base, leader);
+        if ((m >> lane) & 1) slot = base + __popcll(m & ((1ull << lane) - 1));
+        todo &= ~m;
+    }
+    return slot;
+}
+
+__global__ void desc_fill(int32_t* p, int64_t n, int32_t v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void desc_init(Beams b, int n, int B) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    b.nb[r] = 1;
+    b.cat[(int64_t)r * B] = -1;
+}
+
+// key of an entry: its category, n_cat for the root
+__global__ void desc_hist(Beams b, int n, int B, int n_cat, int32_t* hist) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nb = r < n ? b.nb[r] : 0;
+    for (int j = 0; j < B; ++j) {
+        const bool act = j < nb;
+        int key = 0;
+        if (act) {
+            const int c = b.cat[(int64_t)r * B + j];
+            key = c < 0 ? n_cat : c;
+        }
+        (void)wave_agg_add(hist, key, act);
+    }
+}
+
+__global__ void __launch_bounds__(1024)
+desc_scan(const int32_t* hist, int n_keys, int32_t* cursor, int32_t* total) {
+    __shared__ int32_t s[1024];
+    const int t = threadIdx.x;
+    const int per = (n_keys + 1023) / 1024;
+    const int a = min(t * per, n_keys), e = min(a + per, n_keys);
+    int sum = 0;
+    for (int i = a; i < e; ++i) sum += hist[i];
+    s[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int v = t >= off ? s[t - off] : 0;
+        __syncthreads();
+        s[t] += v;
+        __syncthreads();
+    }
+    int run = s[t] - sum;
+    for (int i = a; i < e; ++i) {
+        cursor[i] = run;
+        run += hist[i];
+    }
+    if (t == 1023) *total = s[1023];
+}
+
+__global__ void desc_scatter(Beams b, int n, int B, int n_cat, int32_t* cursor, int32_t* pairs,
+                             int32_t* pkey) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nb = r < n ? b.nb[r] : 0;
+    for (int j = 0; j < B; ++j) {
+        const bool act = j < nb;
+        int key = 0;
+        if (act) {
+            const int c = b.cat[(int64_t)r * B + j];
+            key = c < 0 ? n_cat : c;
+        }
+        const int pos = wave_agg_add(cursor, key, act);
+        if (act) {
+            pairs[pos] = r * B + j;
+            pkey[pos] = key;
+        }
+    }
+}
+
+__device__ __forceinline__ int32_t sdot2(int32_t a, int32_t b, int32_t c) {
+    return __builtin_amdgcn_sdot2(__builtin_bit_cast(short2_t, a), __builtin_bit_cast(short2_t, b),
+                                  c, false);
+}
+
+// Scores pairs [0, *total) of the grouped list.  E: elements per int32
+// partial sum (32, 8 or 2).
+template <int E>
+__global__ void __launch_bounds__(kDescThreads)
+desc_score(TreeDev t, const int16_t* __restrict__ q, int64_t qld, const int64_t* __restrict__ qn,
+           int B, int64_t K, const int32_t* __restrict__ pairs, const int32_t* __restrict__ pkey,
+           const int32_t* __restrict__ total, int Fmax, float* __restrict__ sc) {
+    constexpr int kStep = E > 8 ? E : 8;  // elements per loop trip
+    const int n_pairs = *total;
+    const int gid = blockIdx.x * kDescThreads + threadIdx.x;
+    if ((int)(blockIdx.x * kDescThreads) >= n_pairs) return;  // uniform per workgroup
+    const bool act = gid < n_pairs;
+    const int key = act ? pkey[gid] : -1;
+    const int pr = act ? pairs[gid] : 0;
+    const int r = pr / B;
+    const int lane = lane_id();
+    const int16_t* __restrict__ qrow = q + (int64_t)r * qld;
+    const int64_t qnr = act ? qn[r] : 0;
+    uint64_t todo = __ballot(act);
+    while (todo) {
+        const int leader = __ffsll((unsigned long long)todo) - 1;
+        const int pk = __builtin_amdgcn_readlane(key, leader);  // wave-uniform category
+        const uint64_t m = __ballot(act && key == pk);
+        todo &= ~m;
+        if (!((m >> lane) & 1)) continue;
+        const int32_t* list;
+        int F;
+        span_of(t, pk == t.n_cat ? -1 : pk, list, F);
+        for (int g0 = 0; g0 < F; g0 += kDescG) {
+            const int G = min(kDescG, F - g0);
+            const int16_t* lo[kDescG];
+            const int16_t* hi[kDescG];
+            int cat[kDescG];
+#pragma unroll
+            for (int g = 0; g < kDescG; ++g) {
+                cat[g] = list[g0 + (g < G ? g : G - 1)];
+                lo[g] = t.lo + (int64_t)cat[g] * t.ld;
+                hi[g] = t.hi + (int64_t)cat[g] * t.ld;
+            }
+            int64_t acc[kDescG];
+            int32_t pl[kDescG], ph[kDescG];
+#pragma unroll
+            for (int g = 0; g < kDescG; ++g) { acc[g] = 0; pl[g] = 0; ph[g] = 0; }
+            for (int64_t k = 0; k < K; k += kStep) {
+#pragma unroll
+                for (int s = 0; s < kStep / 8; ++s) {
+                    const int4 qv = *reinterpret_cast<const int4*>(qrow + k + 8 * s);
+                    const int32_t qd[4] = {qv.x, qv.y, qv.z, qv.w};
+#pragma unroll
+                    for (int g = 0; g < kDescG; ++g) {
+                        const int4 lv = *reinterpret_cast<const int4*>(lo[g] + k + 8 * s);
+                        const int4 hv = *reinterpret_cast<const int4*>(hi[g] + k + 8 * s);
+                        const int32_t ld4[4] = {lv.x, lv.y, lv.z, lv.w};
+                        const int32_t hd4[4] = {hv.x, hv.y, hv.z, hv.w};
+#pragma unroll
+                        for (int d = 0; d < 4; ++d) {
+                            pl[g] = sdot2(qd[d], ld4[d], pl[g]);
+                            ph[g] = sdot2(qd[d], hd4[d], ph[g]);
+                            if (E == 2) {
+                                acc[g] += (int64_t)pl[g] + ((int64_t)ph[g] << 16);
+                                pl[g] = 0;
+                                ph[g] = 0;
+                            }
+                        }
+                        if (E == 8) {
+                            acc[g] += (int64_t)pl[g] + ((int64_t)ph[g] << 16);
+                            pl[g] = 0;
+                            ph[g] = 0;
+                        }
+                    }
+                }
+                if (E == 32) {
+#pragma unroll
+                    for (int g = 0; g < kDescG; ++g) {
+                        acc[g] += (int64_t)pl[g] + ((int64_t)ph[g] << 16);
+                        pl[g] = 0;
+                        ph[g] = 0;
+                    }
+                }
+            }
+#pragma unroll
+            for (int g = 0; g < kDescG; ++g)
+                if (g < G)
+                    sc[(int64_t)pr * Fmax + g0 + g] = ref_score(acc[g], qnr, t.sqnorm[cat[g]]);
+        }
+    }
+}
+
+struct DescOut {
+    int32_t* path;   // [n][L]
+    float* pscore;   // [n][L]
+    int32_t* len;    // [n]
+    int32_t* exit;   // [n]
+};
+
+__global__ void desc_select(TreeDev t, Beams cur, Beams nxt, int n, int B, int L, int lvl,
+                            float thr, const float* __restrict__ sc, int Fmax,
+                            float* __restrict__ ws_s, int32_t* __restrict__ ws_c, int maxN,
+                            DescOut out) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const int nb = cur.nb[r];
+    if (nb == 0) {  // exited at an earlier level
+        nxt.nb[r] = 0;
+        return;
+    }
+    float* s = ws_s + (int64_t)r * maxN;
+    int32_t* c = ws_c + (int64_t)r * maxN;
+    int m = 0;
+    for (int j = 0; j < nb; ++j) {
+        const int32_t* list;
+        int F;
+        span_of(t, cur.cat[(int64_t)r * B + j], list, F);
+        const float* row = sc + ((int64_t)r * B + j) * Fmax;
+        for (int f = 0; f < F; ++f) {
+            s[m] = row[f];
+            c[m] = j * Fmax + f;
+            ++m;
+        }
+    }
+    lsort::sort_desc(s, c, m);
+    // candidate id -> (beam entry, category)
+    auto resolve = [&](int id, int& j, int& cat) {
+        j = id / Fmax;
+        const int32_t* list;
+        int F;
+        span_of(t, cur.cat[(int64_t)r * B + j], list, F);
+        cat = list[id - j * Fmax];
+    };
+    if (m == 0 || s[0] < thr) {  // early exit (targets/identify.cpp:186-197)
+        int len = 0;
+        if (m > 0) {
+            int j, cat;
+            resolve(c[0], j, cat);
+            const int64_t src = ((int64_t)r * B + j) * L;
+            for (int x = 0; x < lvl; ++x) {
+                out.path[(int64_t)r * L + x] = cur.path[src + x];
+                out.pscore[(int64_t)r * L + x] = cur.pscore[src + x];
+            }
+            out.path[(int64_t)r * L + lvl] = cat;
+            out.pscore[(int64_t)r * L + lvl] = s[0];
+            len = lvl + 1;
+        }
+        out.len[r] = len;
+        out.exit[r] = lvl;
+        nxt.nb[r] = 0;
+        return;
+    }
+    int keep = 0;
+    while (keep < m && keep < B && s[keep] >= thr) ++keep;
+    for (int i = 0; i < keep; ++i) {
+        int j, cat;
+        resolve(c[i], j, cat);
+        const int64_t src = ((int64_t)r * B + j) * L;
+        const int64_t dst = ((int64_t)r * B + i) * L;
+        for (int x = 0; x < lvl; ++x) {
+            nxt.path[dst + x] = cur.path[src + x];
+            nxt.pscore[dst + x] = cur.pscore[src + x];
+        }
+        nxt.path[dst + lvl] = cat;
+        nxt.pscore[dst + lvl] = s[i];
+        nxt.cat[(int64_t)r * B + i] = cat;
+    }
+    nxt.nb[r] = keep;
+}
+
+// reads that ran every level: the first beam entry (targets/identify.cpp:205-211)
+__global__ void desc_final(Beams cur, int n, int B, int L, int levels, DescOut out) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n || cur.nb[r] == 0) return;
+    const int64_t src = (int64_t)r * B * L;
+    for (int x = 0; x < levels; ++x) {
+        out.path[(int64_t)r * L + x] = cur.path[src + x];
+        out.pscore[(int64_t)r * L + x] = cur.pscore[src + x];
+    }
+    out.len[r] = levels;
+    out.exit[r] = -1;
+}
+
+// ---- prototype construction -------------------------------------------
+
+__global__ void tree_place_kernel(const int16_t* __restrict__ counts, int64_t cld, int64_t ld,
+                                  const int32_t* __restrict__ cat, int32_t* __restrict__ proto) {
+    const int i = blockIdx.x;
+    const int16_t* src = counts + (int64_t)i * cld;
+    int32_t* dst = proto + (int64_t)cat[i] * ld;
+    const int64_t K = cld < ld ? cld : ld;
+    for (int64_t k = threadIdx.x; k < ld; k += blockDim.x) dst[k] = k < K ? (int32_t)src[k] : 0;
+}
+
+__global__ void tree_sum_kernel(int32_t* __restrict__ proto, int64_t ld,
+                                const int32_t* __restrict__ child_off,
+                                const int32_t* __restrict__ child_idx,
+                                const int32_t* __restrict__ parents, int32_t* overflow) {
+    const int p = parents[blockIdx.x];
+    const int a = child_off[p], e = child_off[p + 1];
+    bool bad = false;
+    for (int64_t k = threadIdx.x; k < ld; k += blockDim.x) {
+        int64_t s = 0;
+        for (int j = a; j < e; ++j) s += proto[(int64_t)child_idx[j] * ld + k];
+        bad |= s > INT32_MAX || s < INT32_MIN;
+        proto[(int64_t)p * ld + k] = (int32_t)s;
+    }
+    if (bad && overflow) atomicOr(overflow, 1);
+}
+
+// p = 65536 hi + lo, lo in [-32768, 32767]; a row whose hi would not fit in
+// int16 (p >= 2^31 - 32768) reports 65536 as its max |limb|
+__global__ void tree_limbs_kernel(const int32_t* __restrict__ proto, int64_t ld,
+                                  int16_t* __restrict__ lo, int16_t* __restrict__ hi,
+                                  int64_t* __restrict__ sqnorm, int32_t* __restrict__ maxabs) {
+    __shared__ int64_t s_n[kDescThreads];
+    __shared__ int32_t s_m[kDescThreads];
+    const int i = blockIdx.x;
+    int64_t nrm = 0;
+    int32_t mx = 0;
+    for (int64_t k = threadIdx.x; k < ld; k += blockDim.x) {
+        const int32_t p = proto[(int64_t)i * ld + k];
+        const int32_t l = (int32_t)(int16_t)(p & 0xFFFF);
+        const int64_t h = ((int64_t)p - l) >> 16;
+        lo[(int64_t)i * ld + k] = (int16_t)l;
+        hi[(int64_t)i * ld + k] = (int16_t)h;
+        nrm += (int64_t)p * p;
+        const int32_t al = l < 0 ? -l : l;
+        const int32_t ah = h > 32767 ? 65536 : (int32_t)(h < 0 ? -h : h);
+        mx = max(mx, max(al, ah));
+    }
+    s_n[threadIdx.x] = nrm;
+    s_m[threadIdx.x] = mx;
+    __syncthreads();
+    for (int o = kDescThreads / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            s_n[threadIdx.x] += s_n[threadIdx.x + o];
+            s_m[threadIdx.x] = max(s_m[threadIdx.x], s_m[threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (sqnorm) sqnorm[i] = s_n[0];
+        if (maxabs) maxabs[i] = s_m[0];
+    }
+}
+
+inline size_t align_up(size_t x) { return (x + 255) & ~size_t(255); }
+
+// copies host int32 arrays into one device workspace (in order); returns
+// the device pointers
+int upload_i32(hvk_ctx* ctx, Workspace& ws, std::initializer_list<std::pair<const int32_t*, size_t>> parts,
+               std::vector<const int32_t*>& dev, hipStream_t st) {
+    size_t total = 0;
+    for (auto& p : parts) total += align_up(p.second * sizeof(int32_t));
+    if (int e = ws.ensure(total ? total : 256)) return e;
+    char* base = (char*)ws.ptr;
+    size_t off = 0;
+    dev.clear();
+    for (auto& p : parts) {
+        dev.push_back((const int32_t*)(base + off));
+        if (p.second)
+            HVK_HIP(hipMemcpyAsync(base + off, p.first, p.second * sizeof(int32_t),
+                                   hipMemcpyHostToDevice, st));
+        off += align_up(p.second * sizeof(int32_t));
+    }
+    // the host arrays may be pageable and are the caller's: finish the copies
+    HVK_HIP(hipStreamSynchronize(st));
+    return HVK_OK;
+}
+
+}  // namespace
+}  // namespace hvk
+
+using namespace hvk;
+
+extern "C" int hvk_descend(hvk_ctx* ctx, const hvk_tree* tree, const int16_t* d_q_counts,
+                           const int64_t* d_q_sqnorm, int n_q, int64_t q_ld, int q_maxabs,
+                           int beam, float threshold, int levels, int32_t* d_path,
+                           float* d_path_score, int32_t* d_path_len, int32_t* d_exit_level,
+                           hvk_stream stream) {
+    if (!ctx || !tree || n_q < 0 || beam < 1 || beam > kMaxBeam || levels < 1 ||
+        levels > kMaxLevels || q_maxabs < 0)
+        return HVK_ERR_ARG;
+    if (n_q == 0) return HVK_OK;
+    if (!d_q_counts || !d_q_sqnorm || !d_path || !d_path_score || !d_path_len || !d_exit_level)
+        return HVK_ERR_ARG;
+    const int n_cat = tree->n_cat;
+    if (n_cat < 0 || tree->n_roots < 0 || !tree->child_off || (tree->n_roots && !tree->roots))
+        return HVK_ERR_ARG;
+    if (n_cat > 0 && (!tree->d_lo || !tree->d_hi || !tree->d_sqnorm)) return HVK_ERR_ARG;
+    if (q_ld == 0) q_ld = ctx->ld;
+    const int64_t tld = tree->ld == 0 ? ctx->ld : tree->ld;
+    if (q_ld < ctx->dim || tld < ctx->dim || q_ld % 128 || tld % 128) return HVK_ERR_ARG;
+    // candidate lists: offsets monotone from 0, indices in range
+    if (tree->child_off[0] != 0) return HVK_ERR_ARG;
+    int max_span = 0;
+    for (int c = 0; c < n_cat; ++c) {
+        const int w = tree->child_off[c + 1] - tree->child_off[c];
+        if (w < 0) return HVK_ERR_ARG;
+        max_span = std::max(max_span, w);
+    }
+    const int nnz = tree->child_off[n_cat];
+    if (nnz > 0 && !tree->child_idx) return HVK_ERR_ARG;
+    for (int i = 0; i < nnz; ++i)
+        if (tree->child_idx[i] < 0 || tree->child_idx[i] >= n_cat) return HVK_ERR_ARG;
+    for (int i = 0; i < tree->n_roots; ++i)
+        if (tree->roots[i] < 0 || tree->roots[i] >= n_cat) return HVK_ERR_ARG;
+    // int32 partial sums: E elements of |q| * |limb| each
+    if (tree->max_abs_limb < 0 || tree->max_abs_limb > 32768) return HVK_ERR_RANGE;
+    const int64_t prod = (int64_t)q_maxabs * tree->max_abs_limb;
+    const int64_t lim = int64_t(1) << 31;
+    int E;
+    if (32 * prod < lim) E = 32;
+    else if (8 * prod < lim) E = 8;
+    else if (2 * prod < lim) E = 2;
+    else return HVK_ERR_RANGE;
+
+    const int B = beam, L = levels;
+    const int Fmax = std::max(1, std::max(tree->n_roots, max_span));
+    const int maxN = std::max(1, std::max(tree->n_roots, B * max_span));
+    const int64_t n = n_q;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    hipStream_t st = (hipStream_t)stream;
+
+    std::vector<const int32_t*> dev;
+    if (int e = upload_i32(ctx, ctx->ws_desc_tree,
+                           {{tree->child_off, (size_t)n_cat + 1},
+                            {tree->child_idx, (size_t)nnz},
+                            {tree->roots, (size_t)tree->n_roots}},
+                           dev, st))
+        return e;
+    const TreeDev t{tree->d_lo, tree->d_hi, tree->d_sqnorm, tld, n_cat,
+                    dev[0], dev[1], dev[2], tree->n_roots};
+
+    // workspace: two beam states, scores, grouped pairs, buckets, sort rows
+    const size_t beam_bytes = align_up(n * 4) + align_up(n * B * 4) + 2 * align_up(n * B * L * 4);
+    const size_t sc_bytes = align_up((size_t)n * B * Fmax * 4);
+    const size_t pair_bytes = 2 * align_up((size_t)n * B * 4);
+    const size_t key_bytes = 2 * align_up(((size_t)n_cat + 1) * 4) + 256;
+    const size_t sort_bytes = 2 * align_up((size_t)n * maxN * 4);
+    if (int e = ctx->ws_desc.ensure(2 * beam_bytes + sc_bytes + pair_bytes + key_bytes + sort_bytes))
+        return e;
+    char* w = (char*)ctx->ws_desc.ptr;
+    auto take = [&](size_t bytes) { char* p = w; w += align_up(bytes); return p; };
+    Beams bm[2];
+    for (int s = 0; s < 2; ++s) {
+        bm[s].nb = (int32_t*)take(n * 4);
+        bm[s].cat = (int32_t*)take(n * B * 4);
+        bm[s].path = (int32_t*)take(n * B * L * 4);
+        bm[s].pscore = (float*)take(n * B * L * 4);
+    }
+    float* sc = (float*)take((size_t)n * B * Fmax * 4);
+    int32_t* pairs = (int32_t*)take((size_t)n * B * 4);
+    int32_t* pkey = (int32_t*)take((size_t)n * B * 4);
+    int32_t* hist = (int32_t*)take(((size_t)n_cat + 1) * 4);
+    int32_t* cursor = (int32_t*)take(((size_t)n_cat + 1) * 4);
+    int32_t* total = (int32_t*)take(256);
+    float* ws_s = (float*)take((size_t)n * maxN * 4);
+    int32_t* ws_c = (int32_t*)take((size_t)n * maxN * 4);
+
+    const DescOut out{d_path, d_path_score, d_path_len, d_exit_level};
+    const int rb = (int)((n + kDescThreads - 1) / kDescThreads);
+    const int pb = (int)((n * B + kDescThreads - 1) / kDescThreads);
+    const int n_keys = n_cat + 1;
+    const int64_t K = std::min(q_ld, tld);
+
+    timing_mark(ctx, 1, false, st);
+    hipLaunchKernelGGL(desc_init, dim3(rb), dim3(kDescThreads), 0, st, bm[0], n_q, B);
+    HVK_HIP(hipGetLastError());
+    int cur = 0;
+    for (int lvl = 0; lvl < L; ++lvl) {
+        hipLaunchKernelGGL(desc_fill, dim3((n_keys + kDescThreads - 1) / kDescThreads),
+                           dim3(kDescThreads), 0, st, hist, (int64_t)n_keys, 0);
+        hipLaunchKernelGGL(desc_hist, dim3(rb), dim3(kDescThreads), 0, st, bm[cur], n_q, B, n_cat,
+                           hist);
+        hipLaunchKernelGGL(desc_scan, dim3(1), dim3(1024), 0, st, hist, n_keys, cursor, total);
+        hipLaunchKernelGGL(desc_scatter, dim3(rb), dim3(kDescThreads), 0, st, bm[cur], n_q, B,
+                           n_cat, cursor, pairs, pkey);
+        switch (E) {
+            case 32:
+                hipLaunchKernelGGL(desc_score<32>, dim3(pb), dim3(kDescThreads), 0, st, t,
+                                   d_q_counts, q_ld, d_q_sqnorm, B, K, pairs, pkey, total, Fmax, sc);
+                break;
+            case 8:
+                hipLaunchKernelGGL(desc_score<8>, dim3(pb), dim3(kDescThreads), 0, st, t,
+                                   d_q_counts, q_ld, d_q_sqnorm, B, K, pairs, pkey, total, Fmax, sc);
+                break;
+            default:
+                hipLaunchKernelGGL(desc_score<2>, dim3(pb), dim3(kDescThreads), 0, st, t,
+                                   d_q_counts, q_ld, d_q_sqnorm, B, K, pairs, pkey, total, Fmax, sc);
+        }
+        hipLaunchKernelGGL(desc_select, dim3(rb), dim3(kDescThreads), 0, st, t, bm[cur],
+                           bm[cur ^ 1], n_q, B, L, lvl, threshold, sc, Fmax, ws_s, ws_c, maxN, out);
+        HVK_HIP(hipGetLastError());
+        cur ^= 1;
+    }
+    hipLaunchKernelGGL(desc_final, dim3(rb), dim3(kDescThreads), 0, st, bm[cur], n_q, B, L, L, out);
+    HVK_HIP(hipGetLastError());
+    timing_mark(ctx, 1, true, st);
+    return HVK_OK;
+}
+
+extern "C" int hvk_tree_place(hvk_ctx* ctx, const int16_t* d_counts, int64_t counts_ld, int n,
+                              const int32_t* h_cat, int32_t* d_proto, int64_t ld,
+                              hvk_stream stream) {
+    if (!ctx || n < 0) return HVK_ERR_ARG;
+    if (n == 0) return HVK_OK;
+    if (!d_counts || !h_cat || !d_proto) return HVK_ERR_ARG;
+    if (counts_ld == 0) counts_ld = ctx->ld;
+    if (ld == 0) ld = ctx->ld;
+    if (counts_ld < ctx->dim || ld < ctx->dim) return HVK_ERR_ARG;
+    for (int i = 0; i < n; ++i)
+        if (h_cat[i] < 0) return HVK_ERR_ARG;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<const int32_t*> dev;
+    if (int e = upload_i32(ctx, ctx->ws_desc_tree, {{h_cat, (size_t)n}}, dev, st)) return e;
+    hipLaunchKernelGGL(tree_place_kernel, dim3(n), dim3(kDescThreads), 0, st, d_counts, counts_ld,
+                       ld, dev[0], d_proto);
+    HVK_HIP(hipGetLastError());
+    return HVK_OK;
+}
+
+extern "C" int hvk_tree_sum(hvk_ctx* ctx, int32_t* d_proto, int64_t ld, int n_cat,
+                            const int32_t* h_child_off, const int32_t* h_child_idx,
+                            const int32_t* h_parents, int n_parents, int32_t* d_overflow,
+                            hvk_stream stream) {
+    if (!ctx || n_cat < 0 || n_parents < 0) return HVK_ERR_ARG;
+    if (n_parents == 0) return HVK_OK;
+    if (!d_proto || !h_child_off || !h_parents) return HVK_ERR_ARG;
+    if (ld == 0) ld = ctx->ld;
+    if (ld < ctx->dim || h_child_off[0] != 0) return HVK_ERR_ARG;
+    for (int c = 0; c < n_cat; ++c)
+        if (h_child_off[c + 1] < h_child_off[c]) return HVK_ERR_ARG;
+    const int nnz = h_child_off[n_cat];
+    if (nnz > 0 && !h_child_idx) return HVK_ERR_ARG;
+    for (int i = 0; i < nnz; ++i)
+        if (h_child_idx[i] < 0 || h_child_idx[i] >= n_cat) return HVK_ERR_ARG;
+    // the listed parents are written while their children are read: no
+    // listed parent may be a child of a listed parent (one rank per call)
+    std::vector<char> listed(n_cat, 0);
+    for (int i = 0; i < n_parents; ++i) {
+        const int p = h_parents[i];
+        if (p < 0 || p >= n_cat || listed[p]) return HVK_ERR_ARG;
+        listed[p] = 1;
+    }
+    for (int i = 0; i < n_parents; ++i) {
+        const int p = h_parents[i];
+        for (int j = h_child_off[p]; j < h_child_off[p + 1]; ++j)
+            if (listed[h_child_idx[j]]) return HVK_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> g(ctx->mu);
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<const int32_t*> dev;
+    if (int e = upload_i32(ctx, ctx->ws_desc_tree,
+                           {{h_child_off, (size_t)n_cat + 1}, {h_child_idx, (size_t)nnz},
+                            {h_parents, (size_t)n_parents}},
+                           dev, st))
+        return e;
+    hipLaunchKernelGGL(tree_sum_kernel, dim3(n_parents), dim3(kDescThreads), 0, st, d_proto, ld,
+                       dev[0], dev[1], dev[2], d_overflow);
+    HVK_HIP(hipGetLastError());
+    return HVK_OK;
+}
+
+extern "C" int hvk_tree_limbs(hvk_ctx* ctx, const int32_t* d_proto, int n, int64_t ld,
+                              int16_t* d_lo, int16_t* d_hi, int64_t* d_sqnorm,
+                              int32_t* d_maxabs_limb, hvk_stream stream) {
+    if (!ctx || n < 0) return HVK_ERR_ARG;
+    if (n == 0) return HVK_OK;
+    if (!d_proto || !d_lo || !d_hi) return HVK_ERR_ARG;
+    if (ld == 0) ld = ctx->ld;
+    if (ld < ctx->dim) return HVK_ERR_ARG;
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(tree_limbs_kernel, dim3(n), dim3(kDescThreads), 0, st, d_proto, ld, d_lo,
+                       d_hi, d_sqnorm, d_maxabs_limb);
+    HVK_HIP(hipGetLastError());
+    return HVK_OK;
+}

@@ -56,6 +56,8 @@ struct hvk_ctx {
     hvk::Workspace ws_norms;            // inverse norms
     hvk::Workspace ws_flags;            // rescan queues
     hvk::Workspace ws_search2;          // partial lists of the exact second pass
+    hvk::Workspace ws_desc;             // hierarchical descent: beams, scores, buckets
+    hvk::Workspace ws_desc_tree;        // hierarchical descent: uploaded candidate lists
     std::mutex mu;
     // kernel timing (hvk_ctx_set_timing)
     bool timing = false;

@@ -29,6 +29,7 @@ EXPORTS = [
     "hvk_search_plain", "hvk_search_mfma", "hvk_gen_size", "hvk_gen_fill",
     "hvk_ctx_set_timing", "hvk_ctx_last_timing", "hvk_encode_ex", "hvk_encode_windows",
     "hvk_search_mfma_bounded", "hvk_prepare_counts", "hvk_ctx_set_tuning",
+    "hvk_descend", "hvk_tree_place", "hvk_tree_sum", "hvk_tree_limbs",
 ]
 
 TUNE_SPLITS = 1
@@ -52,6 +53,14 @@ class EncodeOut(ctypes.Structure):
 class Bundles(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in ("clamp8", "lo", "hi", "sqnorm", "maxabs",
                                                "resid1")] + [("n", ctypes.c_int)]
+
+
+class Tree(ctypes.Structure):
+    _fields_ = [("n_cat", ctypes.c_int), ("ld", ctypes.c_int64), ("d_lo", ctypes.c_void_p),
+                ("d_hi", ctypes.c_void_p), ("d_sqnorm", ctypes.c_void_p),
+                ("max_abs_limb", ctypes.c_int32), ("child_off", ctypes.c_void_p),
+                ("child_idx", ctypes.c_void_p), ("roots", ctypes.c_void_p),
+                ("n_roots", ctypes.c_int)]
 
 
 class GenSpec(ctypes.Structure):
@@ -111,6 +120,11 @@ def lib():
     L.hvk_ctx_last_timing.argtypes = [vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]
     L.hvk_gen_size.argtypes = [ctypes.POINTER(GenSpec), ctypes.POINTER(i64), ctypes.POINTER(i64)]
     L.hvk_gen_fill.argtypes = [ctypes.POINTER(GenSpec), vp, vp, vp, vp, vp]
+    L.hvk_descend.argtypes = [vp, ctypes.POINTER(Tree), vp, vp, i32, i64, i32, i32,
+                              ctypes.c_float, i32, vp, vp, vp, vp, vp]
+    L.hvk_tree_place.argtypes = [vp, vp, i64, i32, vp, vp, i64, vp]
+    L.hvk_tree_sum.argtypes = [vp, vp, i64, i32, vp, vp, vp, i32, vp, vp]
+    L.hvk_tree_limbs.argtypes = [vp, vp, i32, i64, vp, vp, vp, vp, vp]
     _lib = L
     return L
 
@@ -382,3 +396,173 @@ def prepare_counts(ctx: Context, counts) -> Encoded:
     check(lib().hvk_prepare_counts(ctx.h, _ptr(counts), n, ctypes.byref(o), _stream()),
           "hvk_prepare_counts")
     return e
+
+
+# ----------------------------------------------------------------------------
+# hierarchical descent
+# ----------------------------------------------------------------------------
+class Taxonomy:
+    """Categories 0..n_cat-1 with candidate lists in CSR form (a category
+    without children lists itself, as identify's get_children does), the
+    level-0 candidates `roots`, and `ranks`: the category ids of each rank,
+    top down.  `leaf_seqs`/`leaf_offsets`: the sequences of the last rank's
+    categories, in ranks[-1] order."""
+
+    def __init__(self, child_off, child_idx, roots, ranks, leaf_seqs=None, leaf_offsets=None):
+        self.child_off = np.ascontiguousarray(child_off, np.int32)
+        self.child_idx = np.ascontiguousarray(child_idx, np.int32)
+        self.roots = np.ascontiguousarray(roots, np.int32)
+        self.ranks = [np.ascontiguousarray(r, np.int32) for r in ranks]
+        self.n_cat = len(self.child_off) - 1
+        self.leaf_seqs = leaf_seqs
+        self.leaf_offsets = leaf_offsets
+
+
+def taxonomy_from_parents(parent):
+    """CSR candidate lists from a parent array (parent[c] = -1 for the top
+    rank); children keep id order, leaves list themselves, roots are the
+    top rank in id order."""
+    parent = np.asarray(parent, np.int64)
+    n = len(parent)
+    has_p = parent >= 0
+    nkids = np.bincount(parent[has_p], minlength=n) if n else np.zeros(0, np.int64)
+    leaf = np.nonzero(nkids == 0)[0]
+    key = np.concatenate([parent[has_p], leaf])
+    val = np.concatenate([np.nonzero(has_p)[0], leaf])
+    o = np.lexsort((val, key))
+    off = np.zeros(n + 1, np.int32)
+    off[1:] = np.cumsum(np.maximum(nkids, 1))
+    depth = np.zeros(n, np.int64)
+    for _ in range(n):
+        nd = np.where(has_p, depth[np.maximum(parent, 0)] + 1, 0)
+        if np.array_equal(nd, depth):
+            break
+        depth = nd
+    ranks = [np.nonzero(depth == d)[0] for d in range(int(depth.max()) + 1)] if n else []
+    return Taxonomy(off, val[o].astype(np.int32), np.nonzero(~has_p)[0].astype(np.int32), ranks)
+
+
+def _mutate(rng, codes, rate):
+    m = rng.random(codes.shape) < rate
+    out = codes.copy()
+    out[m] = (out[m] + rng.integers(1, 4, int(m.sum()), dtype=np.uint8)) % 4
+    return out
+
+
+def make_taxonomy(sizes=(40, 200, 800, 3000, 12000, 60000), seq_len=1500,
+                  rates=(0.20, 0.10, 0.06, 0.04, 0.02, 0.01), n_reads=1000,
+                  read_len=(150, 1600), err=0.01, seed=0, chunk=100_000):
+    """Synthetic taxonomy: rank r has sizes[r] categories, category i of rank
+    r has parent floor(i * sizes[r-1] / sizes[r]); each category's sequence
+    is its parent's (a random root for the top rank) with a fraction
+    rates[r] of substituted bases, so leaves are rates[-1] away from their
+    parent centroid.  Reads: a uniform leaf, a log-uniform length in
+    read_len clamped to the leaf length, a uniform start, `err`
+    substitutions.  Returns (Taxonomy with leaf sequences, reads bytes,
+    read offsets, leaf category of each read)."""
+    rng = np.random.default_rng(seed)
+    acgt = np.frombuffer(b"ACGT", np.uint8)
+    base = np.cumsum([0] + list(sizes))
+    parent = np.full(base[-1], -1, np.int64)
+    seq = rng.integers(0, 4, (1, seq_len), dtype=np.uint8)
+    prev = None
+    for r, n in enumerate(sizes):
+        if r == 0:
+            par_local = np.zeros(n, np.int64)
+        else:
+            par_local = (np.arange(n) * sizes[r - 1]) // n
+            parent[base[r]:base[r + 1]] = base[r - 1] + par_local
+        seq = _mutate(rng, seq[par_local], rates[r])
+        prev = seq
+    leaves = prev  # [sizes[-1]][seq_len] codes
+    tax = taxonomy_from_parents(parent)
+    tax.leaf_seqs = acgt[leaves].tobytes()
+    tax.leaf_offsets = np.arange(sizes[-1] + 1, dtype=np.int64) * seq_len
+    truth = rng.integers(0, sizes[-1], n_reads)
+    lens = np.exp(rng.uniform(np.log(read_len[0]), np.log(read_len[1]), n_reads)).astype(np.int64)
+    lens = np.minimum(lens, seq_len)
+    starts = (rng.random(n_reads) * (seq_len - lens + 1)).astype(np.int64)
+    offs = np.zeros(n_reads + 1, np.int64)
+    offs[1:] = np.cumsum(lens)
+    out = np.empty(int(offs[-1]), np.uint8)
+    for a in range(0, n_reads, chunk):
+        b = min(n_reads, a + chunk)
+        ln = lens[a:b]
+        pos = np.arange(int(offs[b] - offs[a])) - np.repeat(offs[a:b] - offs[a], ln)
+        src = leaves[np.repeat(truth[a:b], ln), np.repeat(starts[a:b], ln) + pos]
+        out[offs[a]:offs[b]] = acgt[_mutate(rng, src, err)]
+    return tax, out.tobytes(), offs, (base[-2] + truth).astype(np.int32)
+
+
+class TreeProto:
+    """Device prototypes of a Taxonomy (int32 rows and their limbs)."""
+
+    def __init__(self, ctx, tax: Taxonomy, proto, lo, hi, sqnorm, maxabs):
+        self.tax, self.proto, self.lo, self.hi, self.sqnorm = tax, proto, lo, hi, sqnorm
+        self.max_abs_limb = int(maxabs.max().item()) if maxabs.numel() else 0
+        self.ld = ctx.ld
+
+    def struct(self) -> Tree:
+        t = self.tax
+        return Tree(t.n_cat, self.ld, self.lo.data_ptr(), self.hi.data_ptr(),
+                    self.sqnorm.data_ptr(), self.max_abs_limb, t.child_off.ctypes.data,
+                    t.child_idx.ctypes.data, t.roots.ctypes.data, len(t.roots))
+
+
+def build_prototypes(ctx: Context, tax: Taxonomy, leaf_counts, leaf_cat,
+                     sum_parents=True) -> TreeProto:
+    """Rows from encoded int16 counts (leaf_counts [n][ld] placed at
+    categories leaf_cat), then, with sum_parents, every category with
+    children set to the sum of its children rank by rank from the bottom,
+    then limbs.  sum_parents=False keeps each placed row as its category's
+    prototype."""
+    torch = _torch()
+    L = lib()
+    n_cat, ld = tax.n_cat, ctx.ld
+    proto = torch.zeros((n_cat, ld), dtype=torch.int32, device="cuda")
+    lc = np.ascontiguousarray(leaf_cat, np.int32)
+    check(L.hvk_tree_place(ctx.h, _ptr(leaf_counts), ld, len(lc), lc.ctypes.data, _ptr(proto), ld,
+                           _stream()), "hvk_tree_place")
+    ovf = torch.zeros(1, dtype=torch.int32, device="cuda")
+    span = np.diff(tax.child_off)
+    self_list = (span == 1) & (tax.child_idx[tax.child_off[:-1]] == np.arange(n_cat))
+    for rank in (reversed(tax.ranks[:-1]) if sum_parents else []):
+        rank = np.ascontiguousarray(rank[~self_list[rank]], np.int32)
+        check(L.hvk_tree_sum(ctx.h, _ptr(proto), ld, n_cat, tax.child_off.ctypes.data,
+                             tax.child_idx.ctypes.data, rank.ctypes.data, len(rank), _ptr(ovf),
+                             _stream()), "hvk_tree_sum")
+    if int(ovf.item()):
+        raise HvkError(HVK_ERR_RANGE, "prototype sums exceed int32")
+    lo = torch.empty((n_cat, ld), dtype=torch.int16, device="cuda")
+    hi = torch.empty_like(lo)
+    sq = torch.empty(n_cat, dtype=torch.int64, device="cuda")
+    mx = torch.empty(n_cat, dtype=torch.int32, device="cuda")
+    check(L.hvk_tree_limbs(ctx.h, _ptr(proto), n_cat, ld, _ptr(lo), _ptr(hi), _ptr(sq), _ptr(mx),
+                           _stream()), "hvk_tree_limbs")
+    return TreeProto(ctx, tax, proto, lo, hi, sq, mx)
+
+
+class Descent:
+    def __init__(self, path, score, length, exit_level):
+        self.path, self.score, self.length, self.exit_level = path, score, length, exit_level
+
+
+def descend(ctx: Context, tree: TreeProto, reads: Encoded, beam=5, threshold=0.0, levels=6,
+            q_maxabs=None, out: Descent | None = None) -> Descent:
+    """identify's beam over the tree's prototypes (hvk_descend); reads need
+    int16 counts (encode(..., counts=True))."""
+    torch = _torch()
+    n = reads.n
+    if out is None:
+        out = Descent(torch.full((n, levels), -1, dtype=torch.int32, device="cuda"),
+                      torch.zeros((n, levels), dtype=torch.float32, device="cuda"),
+                      torch.zeros(n, dtype=torch.int32, device="cuda"),
+                      torch.zeros(n, dtype=torch.int32, device="cuda"))
+    if q_maxabs is None:
+        q_maxabs = int(reads.maxabs.max().item()) if n else 0
+    t = tree.struct()
+    check(lib().hvk_descend(ctx.h, ctypes.byref(t), _ptr(reads.counts), _ptr(reads.sqnorm), n,
+                            ctx.ld, q_maxabs, beam, threshold, levels, _ptr(out.path),
+                            _ptr(out.score), _ptr(out.length), _ptr(out.exit_level), _stream()),
+          "hvk_descend")
+    return out

@@ -217,6 +217,66 @@ int hvk_search_mfma_bounded(hvk_ctx* ctx, const hvk_bundles* q, const hvk_bundle
                             int k, int32_t* d_idx, float* d_score, double* d_cos64,
                             int32_t* d_n_rescored, hvk_stream stream);
 
+/* ---- hierarchical descent ----------------------------------------------
+ * identify's rank-by-rank beam (targets/identify.cpp:89-202) over a taxonomy
+ * whose categories each carry one integer prototype bundle (a leaf's is its
+ * sequence's bundle, a parent's the sum of its children's), scored with the
+ * reference's cosine (src/HV.cpp:219-242) from exact int64 dots:
+ *   score = float(2 * (dot / (sqrt(|q|^2) * sqrt(|p|^2)) - 0.5)), -1 on a zero norm.
+ * Each level expands every beam entry into its candidate list, scores all
+ * candidates, sorts them by score with std::sort's order for equal scores
+ * (comparator a.score > b.score, targets/identify.cpp:183-184), and then
+ * either exits (no candidate, or the best below `threshold`: the read's
+ * result is the best candidate's path, reported with its exit level) or
+ * keeps the first min(beam, #candidates >= threshold) entries.
+ *
+ * Categories are 0..n_cat-1.  Prototype p = 65536*hi + lo with lo in
+ * [-32768, 32767], so |p| < 2^31.  The candidate lists are host arrays
+ * (validated and uploaded by the call): the candidates after category c are
+ * child_idx[child_off[c] .. child_off[c+1]) in order, and a category without
+ * children must list itself (Sequence::get_children returns {path.back()},
+ * targets/identify.cpp:30-35); the level-0 candidates are roots[0..n_roots)
+ * in the order Hierarchy::get_order("p__") returns them
+ * (src/hierarchy.cpp:110-116). */
+typedef struct {
+    int n_cat;
+    int64_t ld;                  /* row stride (elements) of d_lo / d_hi */
+    const int16_t* d_lo;         /* [n_cat][ld] */
+    const int16_t* d_hi;         /* [n_cat][ld] */
+    const int64_t* d_sqnorm;     /* [n_cat] exact sum of p^2 */
+    int32_t max_abs_limb;        /* max |lo|, |hi| over all rows (<= 32768) */
+    const int32_t* child_off;    /* host, [n_cat + 1] */
+    const int32_t* child_idx;    /* host, [child_off[n_cat]] */
+    const int32_t* roots;        /* host, [n_roots] */
+    int n_roots;
+} hvk_tree;
+
+/* Outputs per read (device, caller-allocated): path[levels] category ids and
+ * path_score[levels] scores of the result path, its length, and the level at
+ * which the read exited early (-1 when it ran all levels).  q_maxabs bounds
+ * |q| of every read (the encoder's maxabs).  beam: 1..16 (identify uses 5);
+ * levels: 1..32 (identify walks 6).  HVK_ERR_RANGE when the int32 partial
+ * sums could overflow (|q| * max_abs_limb * 2 must stay below 2^31). */
+int hvk_descend(hvk_ctx* ctx, const hvk_tree* tree, const int16_t* d_q_counts,
+                const int64_t* d_q_sqnorm, int n_q, int64_t q_ld, int q_maxabs, int beam,
+                float threshold, int levels, int32_t* d_path, float* d_path_score,
+                int32_t* d_path_len, int32_t* d_exit_level, hvk_stream stream);
+
+/* Prototype construction.  hvk_tree_place writes the int32 rows
+ * proto[cat[i]] = counts[i] (host cat[n]); hvk_tree_sum sets, for each
+ * parent p listed in h_parents (host, in order), proto[p] = sum of the rows
+ * of its candidate list child_idx[child_off[p] .. child_off[p+1]) (call it
+ * rank by rank from the leaves up); a sum outside int32 sets *d_overflow
+ * (nullable) to 1.  hvk_tree_limbs splits int32 rows into the lo/hi limbs
+ * and exact squared norms hvk_descend reads, and each row's max |limb|. */
+int hvk_tree_place(hvk_ctx* ctx, const int16_t* d_counts, int64_t counts_ld, int n,
+                   const int32_t* h_cat, int32_t* d_proto, int64_t ld, hvk_stream stream);
+int hvk_tree_sum(hvk_ctx* ctx, int32_t* d_proto, int64_t ld, int n_cat, const int32_t* h_child_off,
+                 const int32_t* h_child_idx, const int32_t* h_parents, int n_parents,
+                 int32_t* d_overflow, hvk_stream stream);
+int hvk_tree_limbs(hvk_ctx* ctx, const int32_t* d_proto, int n, int64_t ld, int16_t* d_lo,
+                   int16_t* d_hi, int64_t* d_sqnorm, int32_t* d_maxabs_limb, hvk_stream stream);
+
 /* ---- synthetic workloads (host, deterministic) -------------------------
  * Ancestors of uniform ACGT, descendants by substitution + indels, and reads
  * sampled from descendants with substitution errors; every draw comes from

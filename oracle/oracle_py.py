@@ -46,6 +46,11 @@ def oracle():
         L.orc_precompute_similarity.argtypes = [i32, i32, i32, i32, ctypes.c_char_p, sz,
                                                 ctypes.c_char_p, sz, ctypes.POINTER(sz)]
         L.orc_precompute_windows.argtypes = [i32, i32, i32, i32, ctypes.c_char_p, sz, vp, i32]
+        L.orc_descend.argtypes = [i32, vp, vp, vp, vp, i32, vp, i32, i32, ctypes.c_float, i32,
+                                  vp, vp, vp, vp, i32]
+        L.orc_descend_scores.argtypes = [i32, vp, vp, vp, i32, vp]
+        L.orc_tax_order.argtypes = [ctypes.c_char_p, ctypes.c_char_p, i32, vp, vp, i32, vp,
+                                    ctypes.POINTER(ctypes.c_int)]
         L.orc_classify_reads.restype = ctypes.c_double
         L.orc_classify_reads.argtypes = [i32, i32, ctypes.c_char_p, vp, i32, vp, vp, i32, i32,
                                          vp, vp, i32]
@@ -153,3 +158,38 @@ def precompute_windows(D, k, window, step, ref, max_windows=4096):
     out = np.zeros((max_windows, D), np.int16)
     n = oracle().orc_precompute_windows(D, k, window, step, r, len(r), out.ctypes.data, max_windows)
     return out[:n]
+
+
+def descend(proto, child_off, child_idx, roots, q, beam, thr, levels, threads=8):
+    """identify's beam over per-category prototypes (orc_descend).
+    proto [n_cat][D] int32, q [n][D] int16; returns path, pscore, len, exit."""
+    proto = np.ascontiguousarray(proto, np.int32)
+    q = np.ascontiguousarray(q, np.int16)
+    co = np.ascontiguousarray(child_off, np.int32)
+    ci = np.ascontiguousarray(child_idx, np.int32)
+    ro = np.ascontiguousarray(roots, np.int32)
+    n, D = q.shape
+    path = np.full((n, levels), -1, np.int32)
+    ps = np.zeros((n, levels), np.float32)
+    ln = np.zeros(n, np.int32)
+    ex = np.zeros(n, np.int32)
+    oracle().orc_descend(D, proto.ctypes.data, co.ctypes.data, ci.ctypes.data, ro.ctypes.data,
+                         len(ro), q.ctypes.data, n, beam, thr, levels, path.ctypes.data,
+                         ps.ctypes.data, ln.ctypes.data, ex.ctypes.data, threads)
+    return path, ps, ln, ex
+
+
+def tax_order(tsv_path, names):
+    """Candidate lists (child_off, child_idx) and roots in the order of the
+    reference's Hierarchy containers (orc_tax_order)."""
+    n = len(names)
+    cap = 16 * n + 1024
+    off = np.zeros(n + 1, np.int32)
+    idx = np.zeros(cap, np.int32)
+    roots = np.zeros(max(n, 1), np.int32)
+    nr = ctypes.c_int()
+    m = oracle().orc_tax_order(tsv_path.encode(), "\n".join(names).encode(), n, off.ctypes.data,
+                               idx.ctypes.data, cap, roots.ctypes.data, ctypes.byref(nr))
+    if m < 0:
+        raise ValueError("candidate lists exceed capacity")
+    return off, idx[:m].copy(), roots[: nr.value].copy()
