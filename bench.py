@@ -55,9 +55,12 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-two-limb", action="store_true")
-    ap.add_argument("--workload", choices=["classify", "precision"], default="classify",
+    ap.add_argument("--workload", choices=["classify", "precision", "descent"], default="classify",
                     help="classify: the reads/s line (1M reads x 100k refs, top-5); precision: "
-                         "per-result error vs FP64 at 500k refs, 2M reads of 250 nt, top-10")
+                         "per-result error vs FP64 at 500k refs, 2M reads of 250 nt, top-10; "
+                         "descent: top-3 beam down a 7-rank taxonomy of 60,000 leaves, 1M reads")
+    ap.add_argument("--descent-reads", type=int, default=1_000_000)
+    ap.add_argument("--descent-beam", type=int, default=3)
     ap.add_argument("--precision-reads", type=int, default=2_000_000)
     ap.add_argument("--precision-ancestors", type=int, default=5000)
     ap.add_argument("--copy-calib-mb", type=int, default=0,
@@ -78,6 +81,9 @@ def main():
     torch.cuda.set_device(0)
     if a.workload == "precision":
         run_precision(a)
+        return
+    if a.workload == "descent":
+        run_descent(a)
         return
     if a.copy_calib_mb > 0:
         nbytes = a.copy_calib_mb << 20
@@ -380,6 +386,117 @@ def run_precision(a):
                      "n_rescored": int(res["two_limb"][0].n_rescored.item()),
                      "top10_equal_bounded": same_forms},
         "oracle_sample": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def run_descent(a):
+    """Hierarchical descent: a 7-rank taxonomy (1 root, 40 phyla, 200 classes, 800 orders,
+    3,000 families, 12,000 genera, 60,000 species leaves of 1,500 nt, each 1% from its genus
+    centroid), parent prototypes the integer sums of their children's bundles, 1M reads of
+    150-1,600 nt (log-uniform, clamped to the leaf length) with 1% substitutions; top-3 beam
+    per rank, identify's scoring and tie order, threshold -1 (the integer-sum prototypes of the
+    top ranks score reads below identify's 0.0 cut).  One step = encode the reads + the six
+    levels of the descent; inputs resident in HBM; prototypes built once, outside the step."""
+    import torch
+    import hdc_amd as H
+    t0 = time.time()
+    nq, beam, thr = a.descent_reads, a.descent_beam, -1.0
+    tax, rd, roff, truth = H.make_taxonomy(n_reads=nq, seed=0)
+    print(f"[descent] generated {tax.n_cat} categories, {nq} reads "
+          f"({len(rd) / 1e6:.0f} Mnt) in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    ctx = H.Context(DIM, KMER)
+    ctx.set_timing(True)
+    t1 = time.time()
+    leaf_enc = H.encode(ctx, H.DeviceSeqs(tax.leaf_seqs, tax.leaf_offsets), counts=True,
+                        limbs=False, hi=False)
+    tree = H.build_prototypes(ctx, tax, leaf_enc.counts, tax.ranks[-1])
+    torch.cuda.synchronize()
+    build_s = time.time() - t1
+    del leaf_enc
+    read_seqs = H.DeviceSeqs(rd, roff)
+    L = H.lib()
+    reads = H._alloc(nq, ctx.ld, counts=True, limbs=False, hi=False, clamp=False)
+    reads_out = H._out_struct(reads, ctx.ld)
+
+    def encode_reads():
+        H.check(L.hvk_encode_ex(ctx.h, H._ptr(read_seqs.packed), H._ptr(read_seqs.invalid),
+                                H._ptr(read_seqs.offsets), nq, read_seqs.max_len,
+                                H.ctypes.byref(reads_out), H._stream()), "hvk_encode_ex")
+
+    encode_reads()
+    torch.cuda.synchronize()
+    q_maxabs = int(reads.maxabs.max().item())
+    out = H.Descent(torch.full((nq, 6), -1, dtype=torch.int32, device="cuda"),
+                    torch.zeros((nq, 6), dtype=torch.float32, device="cuda"),
+                    torch.zeros(nq, dtype=torch.int32, device="cuda"),
+                    torch.zeros(nq, dtype=torch.int32, device="cuda"))
+
+    def step():
+        encode_reads()
+        H.descend(ctx, tree, reads, beam=beam, threshold=thr, levels=6, q_maxabs=q_maxabs, out=out)
+
+    for _ in range(a.warmup):
+        step()
+        torch.cuda.synchronize()
+    enc_ms, desc_ms = [], []
+    ts = time.time()
+    for i in range(a.steps):
+        t = time.time()
+        step()
+        torch.cuda.synchronize()
+        e, d = ctx.last_timing()
+        enc_ms.append(e)
+        desc_ms.append(d)
+        print(f"[descent] step {i}: {time.time() - t:.3f}s encode {e:.1f}ms descent {d:.1f}ms",
+              file=sys.stderr, flush=True)
+    sec = (time.time() - ts) / a.steps
+    path = out.path.cpu().numpy()
+    ex = out.exit_level.cpu().numpy()
+    acc = float((path[:, 5] == truth).mean())
+
+    cpu = None
+    if not a.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py as orc
+        ncores = min(16, os.cpu_count() or 1)
+        proto = tree.proto.cpu().numpy()[:, :DIM]
+        qc_all = reads.counts  # device
+        n_cpu = ncores
+        tc = 0.0
+        while True:  # grow the sample until it takes about cpu_seconds
+            qc = qc_all[:n_cpu].cpu().numpy()[:, :DIM]
+            t = time.time()
+            ref = orc.descend(proto, tax.child_off, tax.child_idx, tax.roots, qc, beam, thr, 6,
+                              threads=ncores)
+            tc = time.time() - t
+            if tc * 4 > a.cpu_seconds or n_cpu * 4 > nq:
+                break
+            n_cpu *= 4
+        same = bool(np.array_equal(ref[0], path[:n_cpu]) and
+                    np.array_equal(ref[1].view(np.int32), out.score[:n_cpu].cpu().numpy().view(np.int32)))
+        cpu = {"value": n_cpu / tc, "unit": "reads/s", "cores": ncores, "kind": "port",
+               "sample": f"first {n_cpu} reads (encoded), six levels ({tc:.1f}s wall)",
+               "paths_and_scores_equal_gpu": same}
+        print(f"[descent] cpu: {n_cpu / tc:.1f} reads/s, equal {same}", file=sys.stderr, flush=True)
+
+    # the scoring work: reads x candidates x D multiply-adds (limb passes not counted)
+    n_cand = (len(tax.roots) + beam * 5 + beam * 4 + beam * 3.75 + beam * 4 + beam * 5)
+    line = {
+        "metric": "reads/s classified by hierarchical descent (D=10240, 76,040 categories, "
+                  "top-3 beam per rank)",
+        "value": nq / sec, "unit": "reads/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": 1000.0 * sec, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int16",
+        "data": "synthetic (deterministic generator, seed 0)",
+        "config": {"workload": f"config 4: 7-rank taxonomy 1-40-200-800-3000-12000-60000, "
+                               f"leaves 1500 nt, {nq} reads 150-1600 nt, beam {beam}",
+                   "reads": nq, "categories": tax.n_cat, "beam": beam, "threshold": thr},
+        "encode_ms": float(np.mean(enc_ms)), "descent_ms": float(np.mean(desc_ms)),
+        "prototype_build_s": build_s,
+        "leaf_accuracy": acc, "exited_early": int((ex >= 0).sum()),
+        "approx_int_ops_per_read": 2.0 * DIM * n_cand,
+        "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
 
