@@ -1,0 +1,428 @@
+// batch.hip -- the batching form: length-ordered chunks of reads, each run as
+// encode -> MFMA search -> finalize -> scatter and replayed from a captured
+// hipGraph; an autotuner over the search form, reference slices and chunk
+// size.
+//
+// Reads are ordered by length, longest first (a counting sort of the host
+// offsets), and cut into chunks.  The encoder reads chunk rows through
+// (start, length) pairs in that order, so each launch holds reads of similar
+// length and its workgroups (a static round-robin over sequences) finish
+// together.  The chunk's rows live in batch-owned buffers; after the search,
+// a scatter kernel writes each row's top-k to the read's own output row.
+//
+// Graph capture: the first call that meets a chunk with a given key (its
+// position and size, every input and output pointer, k and the settings)
+// runs the launches directly -- growing the private context's workspaces to
+// their final size -- and then captures the same launches on the batch's
+// stream into a hipGraph; later calls with the same key replay it.  The
+// private context is used by nothing else, so the workspace addresses the
+// graphs hold stay valid.  Nothing in a chunk uses hipMemsetAsync (counters
+// are cleared by kernels).
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "hvk_internal.h"
+
+namespace {
+
+constexpr int kMinChunk = 256;
+
+struct GraphKey {
+    int64_t c0;
+    int n;
+    int k;
+    int form, splits;
+    int max_len;        // longest read of the chunk (selects the encoder's counter width)
+    const void* packed;
+    const void* invalid;
+    hvk_bundles db;
+    const void* idx;
+    const void* score;
+    const void* cos64;
+    const void* nres;
+    const void* order;  // the batch's buffers (move when they grow)
+    const void* rows;
+    const void* outs;
+    const void* ws[4];  // the private context's search workspaces
+    bool operator==(const GraphKey& o) const { return std::memcmp(this, &o, sizeof(*this)) == 0; }
+};
+
+struct Graph {
+    GraphKey key;
+    hipGraphExec_t exec;
+};
+
+}  // namespace
+
+struct hvk_batch {
+    hvk_ctx* ctx = nullptr;
+    int form = HVK_BATCH_FORM_BOUNDED;
+    int splits = 0;
+    int chunk = 0;
+    hipStream_t s = nullptr;
+    hipEvent_t ev_in = nullptr, ev_out = nullptr, t0 = nullptr, t1 = nullptr;
+    hvk::Workspace ws_rows;   // chunk rows and per-row sums
+    hvk::Workspace ws_out;    // chunk outputs
+    hvk::Workspace ws_order;  // perm, starts, lens of the whole batch
+    std::vector<int32_t> perm, lens;
+    std::vector<int64_t> starts;
+    std::vector<int64_t> offsets;  // the offsets the order was computed from
+    std::vector<Graph> graphs;
+};
+
+namespace hvk {
+namespace {
+
+void drop_graphs(hvk_batch* b) {
+    for (auto& g : b->graphs) (void)hipGraphExecDestroy(g.exec);
+    b->graphs.clear();
+}
+
+// out[perm[i]][j] = chunk[i][j]; block 0 adds the chunk's re-scored count
+__global__ void batch_scatter(const int32_t* __restrict__ perm, int n, int k,
+                              const int32_t* __restrict__ ci, const float* __restrict__ cs,
+                              const double* __restrict__ cc, int32_t* __restrict__ oi,
+                              float* __restrict__ os, double* __restrict__ oc,
+                              const int32_t* __restrict__ cnres, int32_t* __restrict__ onres) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t == 0 && onres) atomicAdd(onres, *cnres);
+    if (t >= (int64_t)n * k) return;
+    const int i = (int)(t / k), j = (int)(t - (int64_t)i * k);
+    const int64_t o = (int64_t)perm[i] * k + j;
+    oi[o] = ci[t];
+    os[o] = cs[t];
+    if (oc) oc[o] = cc[t];
+}
+
+__global__ void batch_zero(int32_t* p) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
+}
+
+struct ChunkBufs {
+    int8_t* lo;
+    int8_t* hi;
+    int8_t* clamp8;
+    int64_t* sqnorm;
+    int32_t* maxabs;
+    int32_t* resid1;
+    int32_t* nkmers;
+    int32_t* idx;
+    float* score;
+    double* cos64;
+    int32_t* nres;
+};
+
+inline size_t al(size_t x) { return (x + 255) & ~size_t(255); }
+
+int chunk_bufs(hvk_batch* b, int k, ChunkBufs& cb) {
+    const int64_t ld = b->ctx->ld;
+    const size_t m = (size_t)b->chunk;
+    const size_t rows = 3 * al(m * ld) + al(m * 8) + 3 * al(m * 4);
+    if (int e = b->ws_rows.ensure(rows)) return e;
+    const size_t outs = al(m * k * 4) * 2 + al(m * k * 8) + 256;
+    if (int e = b->ws_out.ensure(outs)) return e;
+    char* p = (char*)b->ws_rows.ptr;
+    auto take = [&](size_t n) { char* q = p; p += al(n); return q; };
+    cb.lo = (int8_t*)take(m * ld);
+    cb.hi = (int8_t*)take(m * ld);
+    cb.clamp8 = (int8_t*)take(m * ld);
+    cb.sqnorm = (int64_t*)take(m * 8);
+    cb.maxabs = (int32_t*)take(m * 4);
+    cb.resid1 = (int32_t*)take(m * 4);
+    cb.nkmers = (int32_t*)take(m * 4);
+    p = (char*)b->ws_out.ptr;
+    cb.idx = (int32_t*)take(m * k * 4);
+    cb.score = (float*)take(m * k * 4);
+    cb.cos64 = (double*)take(m * k * 8);
+    cb.nres = (int32_t*)take(256);
+    return HVK_OK;
+}
+
+// the launches of one chunk, on b->s
+int run_chunk(hvk_batch* b, const uint32_t* packed, const uint32_t* invalid, int64_t c0, int n,
+              int max_len, const hvk_bundles& db, int k, const ChunkBufs& cb, const int32_t* perm,
+              const int64_t* starts, const int32_t* lens, int32_t* oi, float* os, double* oc,
+              int32_t* onres) {
+    hvk_ctx* ctx = b->ctx;
+    const int64_t ld = ctx->ld;
+    EncodeOut eo{};
+    eo.lo = cb.lo;
+    eo.hi = cb.hi;
+    eo.clamp8 = cb.clamp8;
+    eo.ld = ld;
+    eo.sqnorm = cb.sqnorm;
+    eo.maxabs = cb.maxabs;
+    eo.nkmers = cb.nkmers;
+    eo.resid1 = cb.resid1;
+    if (int e = launch_encode(ctx, packed, invalid, nullptr, starts + c0, 0, n, max_len, eo, b->s,
+                              lens + c0))
+        return e;
+    ctx->tune_splits = b->splits;
+    int e;
+    if (b->form == HVK_BATCH_FORM_BOUNDED) {
+        hvk_bundles q{cb.clamp8, cb.lo, cb.hi, cb.sqnorm, cb.maxabs, cb.resid1, n};
+        e = hvk_search_mfma_bounded(ctx, &q, &db, ld, k, cb.idx, cb.score, cb.cos64, cb.nres, b->s);
+    } else {
+        e = hvk_search_mfma(ctx, cb.lo, cb.hi, cb.sqnorm, n, db.lo, db.hi, db.sqnorm, db.n, ld, k,
+                            cb.idx, cb.score, cb.cos64, cb.nres, b->s);
+    }
+    if (e) return e;
+    const int64_t tot = (int64_t)n * k;
+    hipLaunchKernelGGL(batch_scatter, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, b->s,
+                       perm + c0, n, k, cb.idx, cb.score, cb.cos64, oi, os, oc, cb.nres, onres);
+    HVK_HIP(hipGetLastError());
+    return HVK_OK;
+}
+
+// length order (longest first, ties by index) of the reads and their
+// (start, length) pairs, uploaded into ws_order
+int order_reads(hvk_batch* b, const int64_t* h_offsets, int n, int& max_len) {
+    if ((int64_t)b->offsets.size() == (int64_t)n + 1 && b->ws_order.ptr &&
+        std::memcmp(b->offsets.data(), h_offsets, ((size_t)n + 1) * 8) == 0) {
+        max_len = n ? b->lens[0] : 0;  // same reads as the last call: order already uploaded
+        return HVK_OK;
+    }
+    // the host arrays below may still be the source of the last call's copies
+    HVK_HIP(hipStreamSynchronize(b->s));
+    b->offsets.clear();
+    b->perm.resize(n);
+    b->starts.resize(n);
+    b->lens.resize(n);
+    max_len = 0;
+    for (int i = 0; i < n; ++i) {
+        const int64_t l = h_offsets[i + 1] - h_offsets[i];
+        if (l < 0 || l > INT32_MAX) return HVK_ERR_ARG;
+        max_len = std::max<int64_t>(max_len, l);
+    }
+    // counting sort by length, descending, stable
+    std::vector<int64_t> cnt((size_t)max_len + 2, 0);
+    for (int i = 0; i < n; ++i) ++cnt[(size_t)(max_len - (h_offsets[i + 1] - h_offsets[i]))];
+    int64_t run = 0;
+    for (auto& c : cnt) { const int64_t t = c; c = run; run += t; }
+    for (int i = 0; i < n; ++i) {
+        const int64_t l = h_offsets[i + 1] - h_offsets[i];
+        const int64_t pos = cnt[(size_t)(max_len - l)]++;
+        b->perm[pos] = i;
+        b->starts[pos] = h_offsets[i];
+        b->lens[pos] = (int32_t)l;
+    }
+    const size_t need = al((size_t)n * 4) * 2 + al((size_t)n * 8);
+    if (int e = b->ws_order.ensure(need)) return e;
+    char* p = (char*)b->ws_order.ptr;
+    HVK_HIP(hipMemcpyAsync(p, b->perm.data(), (size_t)n * 4, hipMemcpyHostToDevice, b->s));
+    HVK_HIP(hipMemcpyAsync(p + al((size_t)n * 4), b->lens.data(), (size_t)n * 4,
+                           hipMemcpyHostToDevice, b->s));
+    HVK_HIP(hipMemcpyAsync(p + 2 * al((size_t)n * 4), b->starts.data(), (size_t)n * 8,
+                           hipMemcpyHostToDevice, b->s));
+    b->offsets.assign(h_offsets, h_offsets + (size_t)n + 1);
+    return HVK_OK;
+}
+
+int classify(hvk_batch* b, const uint32_t* packed, const uint32_t* invalid,
+             const int64_t* h_offsets, int n, const hvk_bundles& db, int k, int32_t* oi,
+             float* os, double* oc, int32_t* onres, hipStream_t caller) {
+    HVK_HIP(hipEventRecord(b->ev_in, caller));
+    HVK_HIP(hipStreamWaitEvent(b->s, b->ev_in, 0));
+    int max_len = 0;
+    if (int e = order_reads(b, h_offsets, n, max_len)) return e;
+    const char* op = (const char*)b->ws_order.ptr;
+    const int32_t* perm = (const int32_t*)op;
+    const int32_t* lens = (const int32_t*)(op + al((size_t)n * 4));
+    const int64_t* starts = (const int64_t*)(op + 2 * al((size_t)n * 4));
+    ChunkBufs cb;
+    if (int e = chunk_bufs(b, k, cb)) return e;
+    if (onres) {
+        hipLaunchKernelGGL(batch_zero, dim3(1), dim3(64), 0, b->s, onres);
+        HVK_HIP(hipGetLastError());
+    }
+    for (int64_t c0 = 0; c0 < n; c0 += b->chunk) {
+        const int m = (int)std::min<int64_t>(b->chunk, n - c0);
+        const int chunk_max = b->lens[(size_t)c0];  // longest first
+        auto make_key = [&]() {
+            GraphKey key;
+            std::memset(&key, 0, sizeof(key));
+            key.c0 = c0;
+            key.n = m;
+            key.k = k;
+            key.form = b->form;
+            key.splits = b->splits;
+            key.max_len = chunk_max;
+            key.packed = packed;
+            key.invalid = invalid;
+            key.db.clamp8 = db.clamp8;  // field by field: the struct's padding stays zero
+            key.db.lo = db.lo;
+            key.db.hi = db.hi;
+            key.db.sqnorm = db.sqnorm;
+            key.db.maxabs = db.maxabs;
+            key.db.resid1 = db.resid1;
+            key.db.n = db.n;
+            key.idx = oi;
+            key.score = os;
+            key.cos64 = oc;
+            key.nres = onres;
+            key.order = b->ws_order.ptr;
+            key.rows = b->ws_rows.ptr;
+            key.outs = b->ws_out.ptr;
+            key.ws[0] = b->ctx->ws_search.ptr;
+            key.ws[1] = b->ctx->ws_norms.ptr;
+            key.ws[2] = b->ctx->ws_flags.ptr;
+            key.ws[3] = b->ctx->ws_search2.ptr;
+            return key;
+        };
+        GraphKey key = make_key();
+        auto it = std::find_if(b->graphs.begin(), b->graphs.end(),
+                               [&](const Graph& g) { return g.key == key; });
+        if (it != b->graphs.end()) {
+            HVK_HIP(hipGraphLaunch(it->exec, b->s));
+            continue;
+        }
+        // direct run (sizes the workspaces), then capture the same launches
+        if (int e = run_chunk(b, packed, invalid, c0, m, chunk_max, db, k, cb, perm, starts, lens,
+                              oi, os, oc, onres))
+            return e;
+        key = make_key();  // the run may have grown (moved) a workspace
+        HVK_HIP(hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal));
+        const int ce = run_chunk(b, packed, invalid, c0, m, chunk_max, db, k, cb, perm, starts,
+                                 lens, oi, os, oc, onres);
+        hipGraph_t g = nullptr;
+        const hipError_t he = hipStreamEndCapture(b->s, &g);
+        if (ce) {
+            if (g) (void)hipGraphDestroy(g);
+            return ce;
+        }
+        if (he != hipSuccess) { set_last_error("hipStreamEndCapture", he); return HVK_ERR_HIP; }
+        hipGraphExec_t ex = nullptr;
+        const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) { set_last_error("hipGraphInstantiate", ie); return HVK_ERR_HIP; }
+        b->graphs.push_back({key, ex});
+    }
+    HVK_HIP(hipEventRecord(b->ev_out, b->s));
+    HVK_HIP(hipStreamWaitEvent(caller, b->ev_out, 0));
+    return HVK_OK;
+}
+
+int check_inputs(const hvk_batch* b, const uint32_t* packed, const uint32_t* invalid,
+                 const int64_t* h_offsets, int n, const hvk_bundles* db, int k, const int32_t* oi,
+                 const float* os) {
+    if (!b || n < 0 || !db || k < 1 || k > 29) return HVK_ERR_ARG;
+    if (n == 0) return HVK_OK;
+    if (!packed || !invalid || !h_offsets || !oi || !os) return HVK_ERR_ARG;
+    if (!db->clamp8 || !db->lo || !db->sqnorm || !db->maxabs || !db->resid1 || db->n <= 0)
+        return HVK_ERR_ARG;
+    return HVK_OK;
+}
+
+}  // namespace
+}  // namespace hvk
+
+using namespace hvk;
+
+extern "C" int hvk_batch_create(int dim, int kmer, int chunk, hvk_batch** out) {
+    if (!out || chunk < kMinChunk) return HVK_ERR_ARG;
+    *out = nullptr;
+    hvk_ctx* ctx = nullptr;
+    if (int e = hvk_ctx_create(dim, kmer, &ctx)) return e;
+    hvk_batch* b = new hvk_batch;
+    b->ctx = ctx;
+    b->chunk = chunk;
+    hipError_t e = hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreate(&b->t0);
+    if (e == hipSuccess) e = hipEventCreate(&b->t1);
+    if (e != hipSuccess) {
+        set_last_error("hvk_batch_create", e);
+        hvk_batch_destroy(b);
+        return HVK_ERR_HIP;
+    }
+    *out = b;
+    return HVK_OK;
+}
+
+extern "C" int hvk_batch_destroy(hvk_batch* b) {
+    if (!b) return HVK_OK;
+    if (b->s) (void)hipStreamSynchronize(b->s);
+    drop_graphs(b);
+    for (hipEvent_t ev : {b->ev_in, b->ev_out, b->t0, b->t1})
+        if (ev) (void)hipEventDestroy(ev);
+    if (b->s) (void)hipStreamDestroy(b->s);
+    hvk_ctx* ctx = b->ctx;
+    delete b;  // frees the workspaces before the context
+    return hvk_ctx_destroy(ctx);
+}
+
+extern "C" int hvk_batch_config(hvk_batch* b, int form, int splits, int chunk) {
+    if (!b || (form != HVK_BATCH_FORM_BOUNDED && form != HVK_BATCH_FORM_TWO_LIMB) || splits < 0 ||
+        splits > 64 || chunk < kMinChunk)
+        return HVK_ERR_ARG;
+    if (form != b->form || splits != b->splits || chunk != b->chunk) {
+        HVK_HIP(hipStreamSynchronize(b->s));
+        drop_graphs(b);
+    }
+    b->form = form;
+    b->splits = splits;
+    b->chunk = chunk;
+    return HVK_OK;
+}
+
+extern "C" int hvk_batch_get_config(const hvk_batch* b, int* form, int* splits, int* chunk,
+                                    int* n_graphs) {
+    if (!b) return HVK_ERR_ARG;
+    if (form) *form = b->form;
+    if (splits) *splits = b->splits;
+    if (chunk) *chunk = b->chunk;
+    if (n_graphs) *n_graphs = (int)b->graphs.size();
+    return HVK_OK;
+}
+
+extern "C" int hvk_batch_classify(hvk_batch* b, const uint32_t* d_packed,
+                                  const uint32_t* d_invalid, const int64_t* h_offsets, int n,
+                                  const hvk_bundles* db, int k, int32_t* d_idx, float* d_score,
+                                  double* d_cos64, int32_t* d_n_rescored, hvk_stream stream) {
+    if (int e = check_inputs(b, d_packed, d_invalid, h_offsets, n, db, k, d_idx, d_score)) return e;
+    if (n == 0) return HVK_OK;
+    return classify(b, d_packed, d_invalid, h_offsets, n, *db, k, d_idx, d_score, d_cos64,
+                    d_n_rescored, (hipStream_t)stream);
+}
+
+extern "C" int hvk_batch_autotune(hvk_batch* b, const uint32_t* d_packed,
+                                  const uint32_t* d_invalid, const int64_t* h_offsets, int n,
+                                  const hvk_bundles* db, int k, int32_t* d_idx, float* d_score,
+                                  int sample, float* ms_per_mread, hvk_stream stream) {
+    if (int e = check_inputs(b, d_packed, d_invalid, h_offsets, n, db, k, d_idx, d_score)) return e;
+    if (n == 0 || sample <= 0) return HVK_ERR_ARG;
+    const int m = std::min(n, sample);
+    struct Cand { int form, splits, chunk; };
+    std::vector<Cand> cands;
+    for (int chunk : {m / 4, m / 2, m})
+        for (int s : {2, 4, 8})
+            if (chunk >= kMinChunk) cands.push_back({HVK_BATCH_FORM_BOUNDED, s, chunk});
+    cands.push_back({HVK_BATCH_FORM_TWO_LIMB, 4, std::max(kMinChunk, m / 2)});
+    hipStream_t st = (hipStream_t)stream;
+    float best = -1.0f;
+    Cand pick = {b->form, b->splits, b->chunk};
+    for (const Cand& c : cands) {
+        if (int e = hvk_batch_config(b, c.form, c.splits, c.chunk)) return e;
+        // direct run + capture, then the timed replay
+        if (int e = classify(b, d_packed, d_invalid, h_offsets, m, *db, k, d_idx, d_score, nullptr,
+                             nullptr, st))
+            return e;
+        HVK_HIP(hipEventRecord(b->t0, st));
+        if (int e = classify(b, d_packed, d_invalid, h_offsets, m, *db, k, d_idx, d_score, nullptr,
+                             nullptr, st))
+            return e;
+        HVK_HIP(hipEventRecord(b->t1, st));
+        HVK_HIP(hipEventSynchronize(b->t1));
+        float ms = 0.0f;
+        HVK_HIP(hipEventElapsedTime(&ms, b->t0, b->t1));
+        if (best < 0.0f || ms < best) {
+            best = ms;
+            pick = c;
+        }
+    }
+    if (int e = hvk_batch_config(b, pick.form, pick.splits, pick.chunk)) return e;
+    if (ms_per_mread) *ms_per_mread = best * 1e6f / (float)m;
+    return HVK_OK;
+}

@@ -99,7 +99,7 @@ __global__ void __launch_bounds__(1024)
 encode_kernel(const uint32_t* __restrict__ gtable, int k_rt, int D, int W32,
               const uint32_t* __restrict__ packed, const uint32_t* __restrict__ invalid,
               const int64_t* __restrict__ offsets, const int64_t* __restrict__ starts,
-              int window_len, int n_seqs, EncodeOut out) {
+              const int32_t* __restrict__ lens, int window_len, int n_seqs, EncodeOut out) {
     const int k = K > 0 ? K : k_rt;
     extern __shared__ uint32_t smem[];
     uint32_t* tab = smem;
@@ -119,9 +119,11 @@ encode_kernel(const uint32_t* __restrict__ gtable, int k_rt, int D, int W32,
     const int kW = k * W32;
 
     for (int s = blockIdx.x; s < n_seqs; s += gridDim.x) {
-        // sequence s: [offsets[s], offsets[s+1]) or the window starting at starts[s]
+        // sequence s: [offsets[s], offsets[s+1]), or lens[s] (else window_len)
+        // bases from starts[s]
         const int64_t off = starts ? starts[s] : offsets[s];
-        const int64_t len = starts ? (int64_t)window_len : offsets[s + 1] - off;
+        const int64_t len = starts ? (lens ? (int64_t)lens[s] : (int64_t)window_len)
+                                   : offsets[s + 1] - off;
         const int64_t npos = len - k + 1;
         uint32_t planes[P];
 #pragma unroll
@@ -259,8 +261,8 @@ encode_kernel(const uint32_t* __restrict__ gtable, int k_rt, int D, int W32,
 
 template <int K, int P>
 int launch_kp(hvk_ctx* ctx, const uint32_t* packed, const uint32_t* invalid,
-              const int64_t* offsets, const int64_t* starts, int window_len, int n,
-              const EncodeOut& out, hipStream_t st) {
+              const int64_t* offsets, const int64_t* starts, const int32_t* lens, int window_len,
+              int n, const EncodeOut& out, hipStream_t st) {
     const int W32 = ctx->w32;
     const int threads = (W32 + 63) / 64 * 64;
     const size_t lds = sizeof(uint32_t) * ((size_t)4 * ctx->k * W32 + kChunk);
@@ -268,8 +270,8 @@ int launch_kp(hvk_ctx* ctx, const uint32_t* packed, const uint32_t* invalid,
     int grid = n < 4 * ctx->n_cu ? n : 4 * ctx->n_cu;
     timing_mark(ctx, 0, false, st);
     hipLaunchKernelGGL((encode_kernel<K, P>), dim3(grid), dim3(threads), lds, st, ctx->d_table,
-                       ctx->k, ctx->dim, W32, packed, invalid, offsets, starts, window_len, n,
-                       out);
+                       ctx->k, ctx->dim, W32, packed, invalid, offsets, starts, lens, window_len,
+                       n, out);
     timing_mark(ctx, 0, true, st);
     HVK_HIP(hipGetLastError());
     return HVK_OK;
@@ -277,26 +279,26 @@ int launch_kp(hvk_ctx* ctx, const uint32_t* packed, const uint32_t* invalid,
 
 template <int K>
 int launch_k(hvk_ctx* ctx, int max_len, const uint32_t* packed, const uint32_t* invalid,
-             const int64_t* offsets, const int64_t* starts, int window_len, int n,
-             const EncodeOut& out, hipStream_t st) {
+             const int64_t* offsets, const int64_t* starts, const int32_t* lens, int window_len,
+             int n, const EncodeOut& out, hipStream_t st) {
     const int64_t nmax = (int64_t)max_len - ctx->k + 1;
     if (nmax < (1 << 11))
-        return launch_kp<K, 11>(ctx, packed, invalid, offsets, starts, window_len, n, out, st);
+        return launch_kp<K, 11>(ctx, packed, invalid, offsets, starts, lens, window_len, n, out, st);
     if (nmax < (1 << 16))
-        return launch_kp<K, 16>(ctx, packed, invalid, offsets, starts, window_len, n, out, st);
-    return launch_kp<K, 24>(ctx, packed, invalid, offsets, starts, window_len, n, out, st);
+        return launch_kp<K, 16>(ctx, packed, invalid, offsets, starts, lens, window_len, n, out, st);
+    return launch_kp<K, 24>(ctx, packed, invalid, offsets, starts, lens, window_len, n, out, st);
 }
 
 }  // namespace
 
 int launch_encode(hvk_ctx* ctx, const uint32_t* packed, const uint32_t* invalid,
                   const int64_t* offsets, const int64_t* starts, int window_len, int n,
-                  int max_len, const EncodeOut& out, hipStream_t st) {
+                  int max_len, const EncodeOut& out, hipStream_t st, const int32_t* lens) {
     switch (ctx->k) {
-        case 5: return launch_k<5>(ctx, max_len, packed, invalid, offsets, starts, window_len, n, out, st);
-        case 10: return launch_k<10>(ctx, max_len, packed, invalid, offsets, starts, window_len, n, out, st);
-        case 12: return launch_k<12>(ctx, max_len, packed, invalid, offsets, starts, window_len, n, out, st);
-        default: return launch_k<0>(ctx, max_len, packed, invalid, offsets, starts, window_len, n, out, st);
+        case 5: return launch_k<5>(ctx, max_len, packed, invalid, offsets, starts, lens, window_len, n, out, st);
+        case 10: return launch_k<10>(ctx, max_len, packed, invalid, offsets, starts, lens, window_len, n, out, st);
+        case 12: return launch_k<12>(ctx, max_len, packed, invalid, offsets, starts, lens, window_len, n, out, st);
+        default: return launch_k<0>(ctx, max_len, packed, invalid, offsets, starts, lens, window_len, n, out, st);
     }
 }
 

@@ -87,9 +87,10 @@ struct EncodeOut {
     int32_t* nkmers;
     int32_t* resid1;
 };
-// Sequences are [offsets[i], offsets[i+1]) or, when starts is set, windows of
-// window_len bases starting at starts[i].
+// Sequences are [offsets[i], offsets[i+1]) or, when starts is set, lens[i]
+// (window_len when lens is null) bases starting at starts[i].
 int launch_encode(hvk_ctx* ctx, const uint32_t* packed, const uint32_t* invalid,
                   const int64_t* offsets, const int64_t* starts, int window_len, int n,
-                  int max_len, const EncodeOut& out, hipStream_t s);
+                  int max_len, const EncodeOut& out, hipStream_t s,
+                  const int32_t* lens = nullptr);
 }

@@ -30,7 +30,12 @@ EXPORTS = [
     "hvk_ctx_set_timing", "hvk_ctx_last_timing", "hvk_encode_ex", "hvk_encode_windows",
     "hvk_search_mfma_bounded", "hvk_prepare_counts", "hvk_ctx_set_tuning",
     "hvk_descend", "hvk_tree_place", "hvk_tree_sum", "hvk_tree_limbs",
+    "hvk_batch_create", "hvk_batch_destroy", "hvk_batch_config", "hvk_batch_get_config",
+    "hvk_batch_classify", "hvk_batch_autotune",
 ]
+
+BATCH_FORM_BOUNDED = 0
+BATCH_FORM_TWO_LIMB = 1
 
 TUNE_SPLITS = 1
 
@@ -125,6 +130,14 @@ def lib():
     L.hvk_tree_place.argtypes = [vp, vp, i64, i32, vp, vp, i64, vp]
     L.hvk_tree_sum.argtypes = [vp, vp, i64, i32, vp, vp, vp, i32, vp, vp]
     L.hvk_tree_limbs.argtypes = [vp, vp, i32, i64, vp, vp, vp, vp, vp]
+    L.hvk_batch_create.argtypes = [i32, i32, i32, ctypes.POINTER(vp)]
+    L.hvk_batch_destroy.argtypes = [vp]
+    L.hvk_batch_config.argtypes = [vp, i32, i32, i32]
+    L.hvk_batch_get_config.argtypes = [vp] + [ctypes.POINTER(ctypes.c_int)] * 4
+    L.hvk_batch_classify.argtypes = [vp, vp, vp, vp, i32, ctypes.POINTER(Bundles), i32, vp, vp,
+                                     vp, vp, vp]
+    L.hvk_batch_autotune.argtypes = [vp, vp, vp, vp, i32, ctypes.POINTER(Bundles), i32, vp, vp,
+                                     i32, ctypes.POINTER(ctypes.c_float), vp]
     _lib = L
     return L
 
@@ -297,7 +310,8 @@ class DeviceSeqs:
         self.max_len = int(lens.max()) if self.n else 0
         self.packed = torch.from_numpy(packed.view(np.int32)).cuda()
         self.invalid = torch.from_numpy(invalid.view(np.int32)).cuda()
-        self.offsets = torch.from_numpy(np.ascontiguousarray(offsets, np.int64)).cuda()
+        self.host_offsets = np.ascontiguousarray(offsets, np.int64)
+        self.offsets = torch.from_numpy(self.host_offsets).cuda()
 
 
 def encode(ctx: Context, seqs: DeviceSeqs, counts=True, limbs=True, hi=True,
@@ -566,3 +580,52 @@ def descend(ctx: Context, tree: TreeProto, reads: Encoded, beam=5, threshold=0.0
                             _ptr(out.score), _ptr(out.length), _ptr(out.exit_level), _stream()),
           "hvk_descend")
     return out
+
+
+# ----------------------------------------------------------------------------
+# batching form
+# ----------------------------------------------------------------------------
+class Batch:
+    """Length-ordered, graph-replayed classification of large read batches
+    (hvk_batch_*); owns a private context of the same D, k."""
+
+    def __init__(self, dim: int, kmer: int, chunk: int = 262144):
+        h = ctypes.c_void_p()
+        check(lib().hvk_batch_create(dim, kmer, chunk, ctypes.byref(h)), "hvk_batch_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().hvk_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def config(self, form=BATCH_FORM_BOUNDED, splits=0, chunk=262144):
+        check(lib().hvk_batch_config(self.h, form, splits, chunk), "hvk_batch_config")
+
+    def get_config(self):
+        v = [ctypes.c_int() for _ in range(4)]
+        check(lib().hvk_batch_get_config(self.h, *[ctypes.byref(x) for x in v]),
+              "hvk_batch_get_config")
+        return dict(zip(("form", "splits", "chunk", "graphs"), (x.value for x in v)))
+
+    def classify(self, seqs: DeviceSeqs, db: Bundles, k: int, out: TopK | None = None) -> TopK:
+        o = out or _topk_bufs(seqs.n, k)
+        check(lib().hvk_batch_classify(self.h, _ptr(seqs.packed), _ptr(seqs.invalid),
+                                       seqs.host_offsets.ctypes.data, seqs.n, ctypes.byref(db), k,
+                                       _ptr(o.idx), _ptr(o.score), _ptr(o.cos64),
+                                       _ptr(o.n_rescored), _stream()), "hvk_batch_classify")
+        return o
+
+    def autotune(self, seqs: DeviceSeqs, db: Bundles, k: int, out: TopK, sample: int) -> float:
+        ms = ctypes.c_float()
+        check(lib().hvk_batch_autotune(self.h, _ptr(seqs.packed), _ptr(seqs.invalid),
+                                       seqs.host_offsets.ctypes.data, seqs.n, ctypes.byref(db), k,
+                                       _ptr(out.idx), _ptr(out.score), sample, ctypes.byref(ms),
+                                       _stream()), "hvk_batch_autotune")
+        return ms.value

@@ -277,6 +277,52 @@ int hvk_tree_sum(hvk_ctx* ctx, int32_t* d_proto, int64_t ld, int n_cat, const in
 int hvk_tree_limbs(hvk_ctx* ctx, const int32_t* d_proto, int n, int64_t ld, int16_t* d_lo,
                    int16_t* d_hi, int64_t* d_sqnorm, int32_t* d_maxabs_limb, hvk_stream stream);
 
+/* ---- batching form -------------------------------------------------------
+ * No counterpart in the reference, whose reads go one at a time through
+ * HVCache::get and cosine (targets/identify.cpp:150-157).  Classifies a large
+ * batch of reads of uneven length against a database, top-k, with results
+ * identical to hvk_encode_ex followed by hvk_search_mfma_bounded:
+ *  - reads are ordered by length, longest first, and cut into chunks of at
+ *    most `chunk` reads, so every encoder launch holds reads of similar
+ *    length (its workgroups finish together);
+ *  - a chunk runs encode -> MFMA search -> finalize -> scatter of its rows'
+ *    results to the reads' own rows.  The first call that meets a chunk
+ *    (same position, size, buffers and settings) runs it directly and
+ *    captures the same launches into a hipGraph; later calls replay it;
+ *  - hvk_batch_autotune times candidate settings (search form: bounded one
+ *    limb on 256x256 tiles or two limbs on 256x128 / 128x128 tiles;
+ *    reference slices per read tile; chunk size) on a sample of the batch and
+ *    keeps the fastest.  Results never depend on the settings.
+ * A batch owns a private context for D, k, whose workspaces back the
+ * captured graphs; `db` may come from any context of the same D, k and is
+ * read, not copied: it must stay valid and unchanged while graphs that use
+ * it are replayed.  One batch must not be used from two threads at once. */
+typedef struct hvk_batch hvk_batch;
+#define HVK_BATCH_FORM_BOUNDED  0
+#define HVK_BATCH_FORM_TWO_LIMB 1
+int hvk_batch_create(int dim, int kmer, int chunk, hvk_batch** out);
+int hvk_batch_destroy(hvk_batch* b);
+/* form: HVK_BATCH_FORM_*; splits: 0 = automatic, else 1..64; chunk >= 256.
+ * Changing a setting drops the captured graphs. */
+int hvk_batch_config(hvk_batch* b, int form, int splits, int chunk);
+int hvk_batch_get_config(const hvk_batch* b, int* form, int* splits, int* chunk, int* n_graphs);
+/* d_packed / d_invalid: hvk_pack layout of all reads (device); h_offsets:
+ * n+1 base offsets (host).  Outputs [n][k] (device); d_cos64 and
+ * d_n_rescored (the number of reads re-scored by the exact passes, summed
+ * over chunks) are nullable.  k: 1..16.  Enqueued on `stream`. */
+int hvk_batch_classify(hvk_batch* b, const uint32_t* d_packed, const uint32_t* d_invalid,
+                       const int64_t* h_offsets, int n, const hvk_bundles* db, int k,
+                       int32_t* d_idx, float* d_score, double* d_cos64, int32_t* d_n_rescored,
+                       hvk_stream stream);
+/* Times each candidate setting on the first min(sample, n) reads of the
+ * batch (one direct run, then the timed graph replay, into the caller's
+ * output buffers) and applies the fastest; *ms_per_mread (nullable) gets its
+ * time per million reads.  Synchronises `stream`. */
+int hvk_batch_autotune(hvk_batch* b, const uint32_t* d_packed, const uint32_t* d_invalid,
+                       const int64_t* h_offsets, int n, const hvk_bundles* db, int k,
+                       int32_t* d_idx, float* d_score, int sample, float* ms_per_mread,
+                       hvk_stream stream);
+
 /* ---- synthetic workloads (host, deterministic) -------------------------
  * Ancestors of uniform ACGT, descendants by substitution + indels, and reads
  * sampled from descendants with substitution errors; every draw comes from

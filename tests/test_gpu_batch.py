@@ -1,0 +1,113 @@
+"""Batching form (hvk_batch_*): length-ordered chunks replayed from hipGraphs
+must give exactly the results of the direct path (hvk_encode_ex +
+hvk_search_mfma_bounded), and those equal the oracle's FP64 brute force.
+
+Reads of uneven length (log-uniform 100-1,600 nt, clamped to the source
+reference) plus edge reads (empty, shorter than k, all N) go through several
+chunk sizes, both search forms, first call (direct run + capture) and later
+calls (graph replay, replayed several times with eager work and
+synchronisations between), and the autotuner.
+"""
+import numpy as np
+import pytest
+
+import oracle_py as orc
+
+pytestmark = pytest.mark.gpu
+
+DIM, K, TOPK = 10240, 12, 5
+
+
+@pytest.fixture(scope="module")
+def work():
+    import hdc_amd as H
+    import torch
+    spec = H.make_spec(n_ancestors=10, desc_per_ancestor=20, anc_len_min=1300, anc_len_max=1600,
+                       div_min=0.02, div_max=0.08, n_reads=1500, read_len_min=100,
+                       read_len_max=1600, read_len_log_uniform=1, err_min=0.01, err_max=0.01)
+    w = H.generate(spec)
+    reads = [w.reads[w.read_offsets[i]:w.read_offsets[i + 1]] for i in range(len(w.read_offsets) - 1)]
+    reads += [b"", b"ACGTACGTAC", b"N" * 400, reads[3].lower()]
+    buf, off = H.concat([r.decode() for r in reads])
+    ctx = H.Context(DIM, K)
+    db = H.encode(ctx, H.DeviceSeqs(w.refs, w.ref_offsets), counts=True, limbs=True, hi=True,
+                  clamp=True)
+    seqs = H.DeviceSeqs(buf, off)
+    q = H.encode(ctx, seqs, counts=True, limbs=True, hi=True, clamp=True)
+    direct = H.search_mfma_bounded(ctx, q, db, TOPK)
+    torch.cuda.synchronize()
+    return ctx, db, seqs, q, direct
+
+
+def _same(a, b):
+    import torch
+    torch.cuda.synchronize()
+    assert torch.equal(a.idx, b.idx)
+    assert torch.equal(a.score, b.score)
+    assert torch.equal(a.cos64, b.cos64)
+
+
+def test_direct_path_matches_oracle(work):
+    ctx, db, seqs, q, direct = work
+    qc = q.counts.cpu().numpy()[:, :DIM]
+    rc = db.counts.cpu().numpy()[:, :DIM]
+    ref_idx, ref_cos = orc.topk16(qc, rc, TOPK)
+    assert np.array_equal(direct.idx.cpu().numpy(), ref_idx)
+
+
+@pytest.mark.parametrize("chunk,form,splits", [(256, 0, 0), (512, 0, 2), (1000, 0, 8),
+                                               (4096, 0, 0), (700, 1, 4)])
+def test_batch_equals_direct(work, chunk, form, splits):
+    import hdc_amd as H
+    import torch
+    ctx, db, seqs, q, direct = work
+    print(f"batch chunk={chunk} form={form} splits={splits} n={seqs.n}", flush=True)
+    b = H.Batch(DIM, K, chunk)
+    b.config(form, splits, chunk)
+    dbb = H.bundles(db)
+    out = b.classify(seqs, dbb, TOPK)  # direct runs + captures
+    _same(out, direct)
+    n_chunks = (seqs.n + chunk - 1) // chunk
+    assert b.get_config()["graphs"] == n_chunks
+    for rep in range(3):  # replays, with eager work and syncs in between
+        out.idx.fill_(-7)
+        out.score.fill_(0.0)
+        torch.cuda.synchronize()
+        _ = (q.sqnorm * 2).sum().item()
+        out = b.classify(seqs, dbb, TOPK, out=out)
+        _same(out, direct)
+    assert b.get_config()["graphs"] == n_chunks  # nothing re-captured
+    b.close()
+
+
+def test_batch_new_outputs_and_k(work):
+    """Other output buffers or another k are other graphs; results stay exact."""
+    import hdc_amd as H
+    ctx, db, seqs, q, direct = work
+    b = H.Batch(DIM, K, 512)
+    dbb = H.bundles(db)
+    o1 = b.classify(seqs, dbb, TOPK)
+    o2 = b.classify(seqs, dbb, TOPK)  # fresh buffers: new keys
+    _same(o1, direct)
+    _same(o2, direct)
+    d10 = H.search_mfma_bounded(ctx, q, db, 10)
+    o3 = b.classify(seqs, dbb, 10)
+    _same(o3, d10)
+    o1 = b.classify(seqs, dbb, TOPK, out=o1)  # replay of the first graphs
+    _same(o1, direct)
+    b.close()
+
+
+def test_batch_autotune(work):
+    import hdc_amd as H
+    ctx, db, seqs, q, direct = work
+    b = H.Batch(DIM, K, 512)
+    dbb = H.bundles(db)
+    out = H._topk_bufs(seqs.n, TOPK)
+    ms = b.autotune(seqs, dbb, TOPK, out, sample=1024)
+    cfg = b.get_config()
+    print("autotune", cfg, ms, flush=True)
+    assert ms > 0 and cfg["chunk"] >= 256
+    out = b.classify(seqs, dbb, TOPK, out=out)
+    _same(out, direct)
+    b.close()
