@@ -55,10 +55,14 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-two-limb", action="store_true")
-    ap.add_argument("--workload", choices=["classify", "precision", "descent"], default="classify",
+    ap.add_argument("--workload", choices=["classify", "precision", "descent", "batch"],
+                    default="classify",
                     help="classify: the reads/s line (1M reads x 100k refs, top-5); precision: "
                          "per-result error vs FP64 at 500k refs, 2M reads of 250 nt, top-10; "
-                         "descent: top-3 beam down a 7-rank taxonomy of 60,000 leaves, 1M reads")
+                         "descent: top-3 beam down a 7-rank taxonomy of 60,000 leaves, 1M reads; "
+                         "batch: the batching form on 4M reads of 100-1,600 nt, 100k refs, top-5")
+    ap.add_argument("--batch-reads", type=int, default=4_000_000)
+    ap.add_argument("--batch-sample", type=int, default=262144, help="autotuner sample")
     ap.add_argument("--descent-reads", type=int, default=1_000_000)
     ap.add_argument("--descent-beam", type=int, default=3)
     ap.add_argument("--precision-reads", type=int, default=2_000_000)
@@ -84,6 +88,9 @@ def main():
         return
     if a.workload == "descent":
         run_descent(a)
+        return
+    if a.workload == "batch":
+        run_batch(a)
         return
     if a.copy_calib_mb > 0:
         nbytes = a.copy_calib_mb << 20
@@ -496,6 +503,113 @@ def run_descent(a):
         "prototype_build_s": build_s,
         "leaf_accuracy": acc, "exited_early": int((ex >= 0).sum()),
         "approx_int_ops_per_read": 2.0 * DIM * n_cand,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def run_batch(a):
+    """Batching form: 100k references (2,000 ancestors x 50), 4M reads of log-uniform length
+    100-1,600 nt (clamped to the source reference) with 1% substitutions, top-5.  Reads are
+    ordered by length into chunks, each chunk's encode -> MFMA search -> finalize -> scatter is
+    replayed from a captured hipGraph; the autotuner picks form / slices / chunk on a sample.
+    One step = one hvk_batch_classify over all reads (graph replays), inputs resident in HBM."""
+    import torch
+    import hdc_amd as H
+    t0 = time.time()
+    nq = a.batch_reads
+    spec = H.make_spec(seed=0, n_ancestors=a.ancestors, desc_per_ancestor=50, anc_len_min=1300,
+                       anc_len_max=1600, div_min=0.02, div_max=0.08, indel_rate=0.01,
+                       n_reads=nq, read_len_min=100, read_len_max=1600, read_len_log_uniform=1,
+                       err_min=0.01, err_max=0.01)
+    w = H.generate(spec)
+    n_refs = len(w.ref_offsets) - 1
+    lens = np.diff(w.read_offsets)
+    print(f"[batch] generated {n_refs} refs, {nq} reads (mean {lens.mean():.0f} nt) in "
+          f"{time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    ctx = H.Context(DIM, KMER)
+    db = H.encode(ctx, H.DeviceSeqs(w.refs, w.ref_offsets), counts=True, limbs=True, hi=True,
+                  clamp=True)
+    dbb = H.bundles(db)
+    seqs = H.DeviceSeqs(w.reads, w.read_offsets)
+    torch.cuda.synchronize()
+    b = H.Batch(DIM, KMER, 524288)
+    out = H._topk_bufs(nq, TOPK)
+    t1 = time.time()
+    ms_per_m = b.autotune(seqs, dbb, TOPK, out, a.batch_sample)
+    cfg = b.get_config()
+    print(f"[batch] autotune {time.time() - t1:.1f}s: {cfg} ({ms_per_m:.0f} ms per M reads)",
+          file=sys.stderr, flush=True)
+    for i in range(max(1, a.warmup)):  # direct runs + captures
+        b.classify(seqs, dbb, TOPK, out=out)
+        torch.cuda.synchronize()
+    ts = time.time()
+    for i in range(a.steps):
+        t = time.time()
+        b.classify(seqs, dbb, TOPK, out=out)
+        torch.cuda.synchronize()
+        print(f"[batch] step {i}: {time.time() - t:.3f}s ({b.get_config()['graphs']} graphs)",
+              file=sys.stderr, flush=True)
+    sec = (time.time() - ts) / a.steps
+
+    # the direct path (one encode + one bounded search, reads in input order) on a subset:
+    # same results, and its rate
+    nd = min(nq, 1_000_000)
+    sub = H.DeviceSeqs(w.reads[: w.read_offsets[nd]], w.read_offsets[: nd + 1])
+    ctx.set_timing(True)
+    q = H.encode(ctx, sub, counts=False, limbs=True, hi=True, clamp=True)
+    direct = H._topk_bufs(nd, TOPK)
+    H.search_mfma_bounded(ctx, q, db, TOPK, out=direct)
+    torch.cuda.synchronize()
+    td = time.time()
+    q = H.encode(ctx, sub, counts=False, limbs=True, hi=True, clamp=True)
+    H.search_mfma_bounded(ctx, q, db, TOPK, out=direct)
+    torch.cuda.synchronize()
+    direct_s = time.time() - td
+    same = bool(torch.equal(direct.idx, out.idx[:nd]) and torch.equal(direct.score, out.score[:nd]))
+    del q
+
+    cpu = None
+    if not a.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py as orc
+        ncores = min(16, os.cpu_count() or 1)
+        db_host = np.ascontiguousarray(db.counts.cpu().numpy()[:, :DIM])
+        dbn = db.sqnorm.cpu().numpy()
+        O = orc.oracle()
+
+        def cpu_run(n):
+            oi = np.zeros((n, TOPK), np.int32)
+            oc = np.zeros((n, TOPK), np.float64)
+            offs = np.ascontiguousarray(w.read_offsets[: n + 1])
+            t = O.orc_classify_reads(DIM, KMER, w.reads[: w.read_offsets[n]], offs.ctypes.data, n,
+                                     db_host.ctypes.data, dbn.ctypes.data, n_refs, TOPK,
+                                     oi.ctypes.data, oc.ctypes.data, ncores)
+            return t, oi
+
+        t1, _ = cpu_run(ncores)
+        n_cpu = int(max(ncores, min(nq, (a.cpu_seconds / (t1 / ncores)) // ncores * ncores)))
+        tc, oi = cpu_run(n_cpu)
+        cpu = {"value": n_cpu / tc, "unit": "reads/s", "cores": ncores, "kind": "port",
+               "sample": f"first {n_cpu} reads x all {n_refs} refs, top-5 ({tc:.1f}s wall)",
+               "top5_equal_gpu": bool(np.array_equal(oi, out.idx[:n_cpu].cpu().numpy()))}
+        print(f"[batch] cpu: {n_cpu / tc:.1f} reads/s", file=sys.stderr, flush=True)
+
+    line = {
+        "metric": "reads/s classified, batching form (D=10240, 100k refs, 4M reads of "
+                  "100-1,600 nt, top-5)",
+        "value": nq / sec, "unit": "reads/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": 1000.0 * sec, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int8", "data": "synthetic (deterministic generator, seed 0)",
+        "config": {"workload": f"config 3: D={DIM}, k={KMER}, {n_refs} refs, {nq} reads "
+                               f"log-uniform 100-1600 nt (mean {lens.mean():.0f}), 1% error, "
+                               f"top-{TOPK}", "reads": nq, "refs": n_refs, "topk": TOPK},
+        "autotune": {"form": ["bounded", "two_limb"][cfg["form"]], "splits": cfg["splits"],
+                     "chunk": cfg["chunk"], "ms_per_mread_sample": ms_per_m,
+                     "sample": a.batch_sample},
+        "graphs": b.get_config()["graphs"],
+        "direct_path": {"reads": nd, "reads_per_s": nd / direct_s, "top5_equal_batch": same},
+        "n_rescored": int(out.n_rescored.item()),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
