@@ -7,13 +7,14 @@
 //   1. bucket count: one key per live (read, beam entry) = its category
 //   2. bucket scan (one workgroup)
 //   3. scatter: (read, entry) pairs grouped by category
-//   4. score: every pair against the candidate list of its category.  The
-//      lanes of a wave that share a category score it together: the
-//      category is made wave-uniform, so its children's limb rows are read
-//      through the scalar cache once per wave, while each lane streams its
-//      own read row.  v_dot2_i32_i16 multiplies int16 counts by the int16
-//      limbs (p = 65536 hi + lo) into int32 partial sums, widened to int64
-//      every E elements, E chosen so no partial can overflow.
+//   4. score: every pair against the candidate list of its category.  A
+//      wave takes 8 consecutive pairs of the grouped list (mostly one
+//      category) and splits K across its lanes, so every load reads 1 KB of
+//      one row; each child slab is loaded once for all of the wave's pairs
+//      of that category.  v_dot2_i32_i16 multiplies int16 counts by the
+//      int16 limbs (p = 65536 hi + lo) into int32 partial sums, widened to
+//      int64 every E elements per lane, E chosen so no partial can overflow;
+//      the lane sums are reduced across the wave.
 //   5. select: per read, gather its candidates in identify's order (beam
 //      order, then each entry's candidate list), sort them exactly as
 //      std::sort does (lsort.h), then exit or keep the beam.
@@ -170,90 +171,123 @@ __device__ __forceinline__ int32_t sdot2(int32_t a, int32_t b, int32_t c) {
                                   c, false);
 }
 
-// Scores pairs [0, *total) of the grouped list.  E: elements per int32
-// partial sum (32, 8 or 2).
+// Scores pairs [0, *total) of the grouped list.  E: elements per lane
+// between widenings of the int32 partial sums (8 or 2; a 32-element chunk
+// would keep 2*P*G more partials live and halve the waves per SIMD).
+//
+// One wave takes kDescP consecutive pairs of the grouped list and splits K
+// across its lanes: lane l holds elements [8l, 8l+8) of every 512-element
+// step, so each load instruction reads 1 KB of one row (one page, fully
+// coalesced) -- a lane-per-pair layout would touch 64 rows, and at the deep
+// levels 64 pages, per instruction.  The wave walks the distinct categories
+// of its pairs (wave-uniform), kDescG children at a time; each child slab is
+// loaded once for all the wave's pairs of that category.  The per-lane int64
+// sums are reduced across the wave at the end of each pass.
+constexpr int kDescP = 8;                 // pairs per wave
+constexpr int kDescStep = 64 * 8;         // elements per wave step
+
+__device__ __forceinline__ int64_t wave_sum_i64(int64_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
 template <int E>
 __global__ void __launch_bounds__(kDescThreads)
 desc_score(TreeDev t, const int16_t* __restrict__ q, int64_t qld, const int64_t* __restrict__ qn,
            int B, int64_t K, const int32_t* __restrict__ pairs, const int32_t* __restrict__ pkey,
            const int32_t* __restrict__ total, int Fmax, float* __restrict__ sc) {
-    constexpr int kStep = E > 8 ? E : 8;  // elements per loop trip
+    static_assert(E == 8 || E == 2, "partial-sum chunk");
     const int n_pairs = *total;
-    const int gid = blockIdx.x * kDescThreads + threadIdx.x;
-    if ((int)(blockIdx.x * kDescThreads) >= n_pairs) return;  // uniform per workgroup
-    const bool act = gid < n_pairs;
-    const int key = act ? pkey[gid] : -1;
-    const int pr = act ? pairs[gid] : 0;
-    const int r = pr / B;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int base = (blockIdx.x * (kDescThreads / 64) + wave) * kDescP;
+    if (base >= n_pairs) return;  // uniform per wave
     const int lane = lane_id();
-    const int16_t* __restrict__ qrow = q + (int64_t)r * qld;
-    const int64_t qnr = act ? qn[r] : 0;
-    uint64_t todo = __ballot(act);
+    int key[kDescP], pr[kDescP];
+#pragma unroll
+    for (int p = 0; p < kDescP; ++p) {
+        const bool ok = base + p < n_pairs;
+        key[p] = ok ? pkey[base + p] : -1;
+        pr[p] = ok ? pairs[base + p] : 0;
+    }
+    uint32_t todo = 0;
+#pragma unroll
+    for (int p = 0; p < kDescP; ++p) todo |= (key[p] >= 0 ? 1u : 0u) << p;
     while (todo) {
-        const int leader = __ffsll((unsigned long long)todo) - 1;
-        const int pk = __builtin_amdgcn_readlane(key, leader);  // wave-uniform category
-        const uint64_t m = __ballot(act && key == pk);
-        todo &= ~m;
-        if (!((m >> lane) & 1)) continue;
+        const int pk = key[__builtin_ctz(todo)];
+        uint32_t mine = 0;
+#pragma unroll
+        for (int p = 0; p < kDescP; ++p) mine |= (key[p] == pk ? 1u : 0u) << p;
+        todo &= ~mine;
         const int32_t* list;
         int F;
         span_of(t, pk == t.n_cat ? -1 : pk, list, F);
+        const int16_t* qrow[kDescP];
+#pragma unroll
+        for (int p = 0; p < kDescP; ++p) qrow[p] = q + (int64_t)(pr[p] / B) * qld;
         for (int g0 = 0; g0 < F; g0 += kDescG) {
             const int G = min(kDescG, F - g0);
             const int16_t* lo[kDescG];
             const int16_t* hi[kDescG];
-            int cat[kDescG];
 #pragma unroll
             for (int g = 0; g < kDescG; ++g) {
-                cat[g] = list[g0 + (g < G ? g : G - 1)];
-                lo[g] = t.lo + (int64_t)cat[g] * t.ld;
-                hi[g] = t.hi + (int64_t)cat[g] * t.ld;
+                const int c = list[g0 + (g < G ? g : G - 1)];
+                lo[g] = t.lo + (int64_t)c * t.ld;
+                hi[g] = t.hi + (int64_t)c * t.ld;
             }
-            int64_t acc[kDescG];
-            int32_t pl[kDescG], ph[kDescG];
+            int64_t acc[kDescP][kDescG];
+            int32_t pl[kDescP][kDescG], ph[kDescP][kDescG];
 #pragma unroll
-            for (int g = 0; g < kDescG; ++g) { acc[g] = 0; pl[g] = 0; ph[g] = 0; }
-            for (int64_t k = 0; k < K; k += kStep) {
+            for (int p = 0; p < kDescP; ++p)
 #pragma unroll
-                for (int s = 0; s < kStep / 8; ++s) {
-                    const int4 qv = *reinterpret_cast<const int4*>(qrow + k + 8 * s);
-                    const int32_t qd[4] = {qv.x, qv.y, qv.z, qv.w};
+                for (int g = 0; g < kDescG; ++g) { acc[p][g] = 0; pl[p][g] = 0; ph[p][g] = 0; }
+            for (int64_t k = 8 * lane; k < K; k += kDescStep) {
+                int4 qv[kDescP];
 #pragma unroll
-                    for (int g = 0; g < kDescG; ++g) {
-                        const int4 lv = *reinterpret_cast<const int4*>(lo[g] + k + 8 * s);
-                        const int4 hv = *reinterpret_cast<const int4*>(hi[g] + k + 8 * s);
-                        const int32_t ld4[4] = {lv.x, lv.y, lv.z, lv.w};
-                        const int32_t hd4[4] = {hv.x, hv.y, hv.z, hv.w};
+                for (int p = 0; p < kDescP; ++p)
+                    qv[p] = ((mine >> p) & 1) ? *reinterpret_cast<const int4*>(qrow[p] + k)
+                                               : make_int4(0, 0, 0, 0);
+#pragma unroll
+                for (int g = 0; g < kDescG; ++g) {
+                    if (g >= G) continue;  // uniform
+                    const int4 lv = *reinterpret_cast<const int4*>(lo[g] + k);
+                    const int4 hv = *reinterpret_cast<const int4*>(hi[g] + k);
+                    const int32_t l4[4] = {lv.x, lv.y, lv.z, lv.w};
+                    const int32_t h4[4] = {hv.x, hv.y, hv.z, hv.w};
+#pragma unroll
+                    for (int p = 0; p < kDescP; ++p) {
+                        const int32_t q4[4] = {qv[p].x, qv[p].y, qv[p].z, qv[p].w};
 #pragma unroll
                         for (int d = 0; d < 4; ++d) {
-                            pl[g] = sdot2(qd[d], ld4[d], pl[g]);
-                            ph[g] = sdot2(qd[d], hd4[d], ph[g]);
+                            pl[p][g] = sdot2(q4[d], l4[d], pl[p][g]);
+                            ph[p][g] = sdot2(q4[d], h4[d], ph[p][g]);
                             if (E == 2) {
-                                acc[g] += (int64_t)pl[g] + ((int64_t)ph[g] << 16);
-                                pl[g] = 0;
-                                ph[g] = 0;
+                                acc[p][g] += (int64_t)pl[p][g] + ((int64_t)ph[p][g] << 16);
+                                pl[p][g] = 0;
+                                ph[p][g] = 0;
                             }
                         }
                         if (E == 8) {
-                            acc[g] += (int64_t)pl[g] + ((int64_t)ph[g] << 16);
-                            pl[g] = 0;
-                            ph[g] = 0;
+                            acc[p][g] += (int64_t)pl[p][g] + ((int64_t)ph[p][g] << 16);
+                            pl[p][g] = 0;
+                            ph[p][g] = 0;
                         }
-                    }
-                }
-                if (E == 32) {
-#pragma unroll
-                    for (int g = 0; g < kDescG; ++g) {
-                        acc[g] += (int64_t)pl[g] + ((int64_t)ph[g] << 16);
-                        pl[g] = 0;
-                        ph[g] = 0;
                     }
                 }
             }
 #pragma unroll
-            for (int g = 0; g < kDescG; ++g)
-                if (g < G)
-                    sc[(int64_t)pr * Fmax + g0 + g] = ref_score(acc[g], qnr, t.sqnorm[cat[g]]);
+            for (int p = 0; p < kDescP; ++p)
+#pragma unroll
+                for (int g = 0; g < kDescG; ++g) {
+                    if (g < G && ((mine >> p) & 1)) {
+                        const int64_t dot = wave_sum_i64(acc[p][g]);
+                        if (lane == 0) {
+                            const int r = pr[p] / B;
+                            const int c = list[g0 + g];
+                            sc[(int64_t)pr[p] * Fmax + g0 + g] = ref_score(dot, qn[r], t.sqnorm[c]);
+                        }
+                    }
+                }
         }
     }
 }
@@ -479,8 +513,7 @@ extern "C" int hvk_descend(hvk_ctx* ctx, const hvk_tree* tree, const int16_t* d_
     const int64_t prod = (int64_t)q_maxabs * tree->max_abs_limb;
     const int64_t lim = int64_t(1) << 31;
     int E;
-    if (32 * prod < lim) E = 32;
-    else if (8 * prod < lim) E = 8;
+    if (8 * prod < lim) E = 8;
     else if (2 * prod < lim) E = 2;
     else return HVK_ERR_RANGE;
 
@@ -529,7 +562,7 @@ extern "C" int hvk_descend(hvk_ctx* ctx, const hvk_tree* tree, const int16_t* d_
 
     const DescOut out{d_path, d_path_score, d_path_len, d_exit_level};
     const int rb = (int)((n + kDescThreads - 1) / kDescThreads);
-    const int pb = (int)((n * B + kDescThreads - 1) / kDescThreads);
+    const int pb = (int)((n * B + kDescP * (kDescThreads / 64) - 1) / (kDescP * (kDescThreads / 64)));
     const int n_keys = n_cat + 1;
     const int64_t K = std::min(q_ld, tld);
 
@@ -545,19 +578,12 @@ extern "C" int hvk_descend(hvk_ctx* ctx, const hvk_tree* tree, const int16_t* d_
         hipLaunchKernelGGL(desc_scan, dim3(1), dim3(1024), 0, st, hist, n_keys, cursor, total);
         hipLaunchKernelGGL(desc_scatter, dim3(rb), dim3(kDescThreads), 0, st, bm[cur], n_q, B,
                            n_cat, cursor, pairs, pkey);
-        switch (E) {
-            case 32:
-                hipLaunchKernelGGL(desc_score<32>, dim3(pb), dim3(kDescThreads), 0, st, t,
-                                   d_q_counts, q_ld, d_q_sqnorm, B, K, pairs, pkey, total, Fmax, sc);
-                break;
-            case 8:
-                hipLaunchKernelGGL(desc_score<8>, dim3(pb), dim3(kDescThreads), 0, st, t,
-                                   d_q_counts, q_ld, d_q_sqnorm, B, K, pairs, pkey, total, Fmax, sc);
-                break;
-            default:
-                hipLaunchKernelGGL(desc_score<2>, dim3(pb), dim3(kDescThreads), 0, st, t,
-                                   d_q_counts, q_ld, d_q_sqnorm, B, K, pairs, pkey, total, Fmax, sc);
-        }
+        if (E == 8)
+            hipLaunchKernelGGL(desc_score<8>, dim3(pb), dim3(kDescThreads), 0, st, t, d_q_counts,
+                               q_ld, d_q_sqnorm, B, K, pairs, pkey, total, Fmax, sc);
+        else
+            hipLaunchKernelGGL(desc_score<2>, dim3(pb), dim3(kDescThreads), 0, st, t, d_q_counts,
+                               q_ld, d_q_sqnorm, B, K, pairs, pkey, total, Fmax, sc);
         hipLaunchKernelGGL(desc_select, dim3(rb), dim3(kDescThreads), 0, st, t, bm[cur],
                            bm[cur ^ 1], n_q, B, L, lvl, threshold, sc, Fmax, ws_s, ws_c, maxN, out);
         HVK_HIP(hipGetLastError());

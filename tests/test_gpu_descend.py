@@ -155,7 +155,7 @@ def test_descend_matches_oracle(synth, beam, thr):
 
 
 def test_descend_narrow_partial_sums(synth):
-    """Claiming larger |q| forces 8- and 2-element int32 partials: same results."""
+    """Claiming larger |q| (up to 20000) forces the 2-element int32 partials: same results."""
     base = _run_both(synth, 3, 0.0)
     for qm in (5000, 20000):
         got, ref = _run_both(synth, 3, 0.0, q_maxabs=qm)
