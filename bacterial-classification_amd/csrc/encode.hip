@@ -8,7 +8,8 @@
 //
 // Design (one workgroup per sequence, persistent over sequences):
 //   * the table rot[b][i][w] (4 x k x W32 words, 61 KB at D=10240, k=12) is
-//     staged in LDS once per workgroup;
+//     staged in LDS once per workgroup -- or, when it fits, the table of base
+//     pairs (encode_pair_kernel, below), which halves the lookups;
 //   * phase 1: lanes compute the canonical code of each k-mer position of a
 //     1024-position chunk from the 2-bit packed sequence (rolling-free: one
 //     64-bit window per position), written to LDS as a "digit i = c_i" code;
@@ -94,6 +95,99 @@ __device__ __forceinline__ int wave_max(int v) {
     return v;
 }
 
+// Phase 3, called by every thread of the workgroup: the threads with
+// `active` set own word w of the row and hold its bit-sliced counts; they
+// write counts / limbs / clamped rows, and the workgroup reduces the exact
+// sum of squares, max |count| and clamping residual of row s.
+template <int P>
+__device__ __forceinline__ void finish_row(const uint32_t (&planes)[P], bool active, int w,
+                                           int nvalid, int D, int W32, int64_t s,
+                                           const EncodeOut& out, int64_t* red_sq, int* red_mx,
+                                           int* red_res) {
+    const int tid = threadIdx.x;
+    const int nthr = blockDim.x;
+    const int lane = tid & 63, wave = tid >> 6, nwaves = nthr >> 6;
+    int64_t sq = 0;
+    int mx = 0, res = 0;
+    const int64_t ld = out.ld;
+    if (active) {
+        uint32_t packed16[16];
+        uint32_t plo[8], phi[8], pcl[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) { plo[q] = 0; phi[q] = 0; pcl[q] = 0; }
+#pragma unroll
+        for (int b = 0; b < 32; ++b) {
+            uint32_t pc = 0;
+#pragma unroll
+            for (int p = 0; p < P; ++p) pc |= ((planes[p] >> b) & 1u) << p;
+            int cnt = (32 * w + b < D) ? (int)(int16_t)(nvalid - 2 * (int)pc) : 0;
+            sq += (int64_t)cnt * cnt;
+            const int a = cnt < 0 ? -cnt : cnt;
+            mx = max(mx, a);
+            res += a > 127 ? a - 127 : 0;
+            const uint32_t u16 = (uint32_t)(uint16_t)(int16_t)cnt;
+            if (b & 1) packed16[b >> 1] |= u16 << 16;
+            else packed16[b >> 1] = u16;
+            const int l8 = (int)(int8_t)(cnt & 0xFF);
+            const int h8 = (cnt - l8) >> 8;
+            const int c8 = cnt > 127 ? 127 : (cnt < -127 ? -127 : cnt);
+            plo[b >> 2] |= ((uint32_t)(uint8_t)l8) << (8 * (b & 3));
+            phi[b >> 2] |= ((uint32_t)(uint8_t)h8) << (8 * (b & 3));
+            pcl[b >> 2] |= ((uint32_t)(uint8_t)c8) << (8 * (b & 3));
+        }
+        const int64_t row = s * ld + 32 * (int64_t)w;
+        if (out.counts) {
+            uint4* dst = reinterpret_cast<uint4*>(out.counts + row);
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                dst[q] = make_uint4(packed16[4 * q], packed16[4 * q + 1], packed16[4 * q + 2],
+                                    packed16[4 * q + 3]);
+        }
+        if (out.lo) {
+            uint4* dst = reinterpret_cast<uint4*>(out.lo + row);
+            dst[0] = make_uint4(plo[0], plo[1], plo[2], plo[3]);
+            dst[1] = make_uint4(plo[4], plo[5], plo[6], plo[7]);
+        }
+        if (out.hi) {
+            uint4* dst = reinterpret_cast<uint4*>(out.hi + row);
+            dst[0] = make_uint4(phi[0], phi[1], phi[2], phi[3]);
+            dst[1] = make_uint4(phi[4], phi[5], phi[6], phi[7]);
+        }
+        if (out.clamp8) {
+            uint4* dst = reinterpret_cast<uint4*>(out.clamp8 + row);
+            dst[0] = make_uint4(pcl[0], pcl[1], pcl[2], pcl[3]);
+            dst[1] = make_uint4(pcl[4], pcl[5], pcl[6], pcl[7]);
+        }
+    }
+    // zero padding columns beyond the last full word
+    for (int64_t d = 32 * (int64_t)W32 + tid; d < ld; d += nthr) {
+        const int64_t o = s * ld + d;
+        if (out.counts) out.counts[o] = 0;
+        if (out.lo) out.lo[o] = 0;
+        if (out.hi) out.hi[o] = 0;
+        if (out.clamp8) out.clamp8[o] = 0;
+    }
+    sq = wave_sum64(sq);
+    mx = wave_max(mx);
+    res = wave_sum(res);
+    if (lane == 0) { red_sq[wave] = sq; red_mx[wave] = mx; red_res[wave] = res; }
+    __syncthreads();
+    if (tid == 0) {
+        int64_t t = 0;
+        int m = 0, r = 0;
+        for (int q = 0; q < nwaves; ++q) {
+            t += red_sq[q];
+            m = max(m, red_mx[q]);
+            r += red_res[q];
+        }
+        if (out.sqnorm) out.sqnorm[s] = t;
+        if (out.maxabs) out.maxabs[s] = m;
+        if (out.nkmers) out.nkmers[s] = nvalid;
+        if (out.resid1) out.resid1[s] = r;
+    }
+    __syncthreads();
+}
+
 template <int K, int P>
 __global__ void __launch_bounds__(1024)
 encode_kernel(const uint32_t* __restrict__ gtable, int k_rt, int D, int W32,
@@ -110,7 +204,6 @@ encode_kernel(const uint32_t* __restrict__ gtable, int k_rt, int D, int W32,
 
     const int tid = threadIdx.x;
     const int nthr = blockDim.x;
-    const int lane = tid & 63, wave = tid >> 6, nwaves = nthr >> 6;
     for (int i = tid; i < 4 * k * W32; i += nthr) tab[i] = gtable[i];
 
     const uint32_t mask2k = (k == 16) ? 0xFFFFFFFFu : ((1u << (2 * k)) - 1u);
@@ -176,86 +269,132 @@ encode_kernel(const uint32_t* __restrict__ gtable, int k_rt, int D, int W32,
             }
         }
 
-        // ---- phase 3: per-dimension counts ----
-        int64_t sq = 0;
-        int mx = 0, res = 0;
-        const int64_t ld = out.ld;
-        if (w < W32) {
-            uint32_t packed16[16];
-            uint32_t plo[8], phi[8], pcl[8];
+        finish_row<P>(planes, w < W32, w, nvalid, D, W32, s, out, red_sq, red_mx, red_res);
+    }
+}
+
+// Pair-table encoder: the table of base PAIRS T2[c_2j + 4 c_2j+1][j][w] =
+// rot(base[c_2j], 2j) ^ rot(base[c_2j+1], 2j+1) halves the LDS lookups and
+// address adds per k-mer (6 instead of 12 at k = 12).  The table is 4x the
+// single one (123 KB at D = 10240, k = 12), so one workgroup fills a CU; to
+// keep ten waves per CU, two halves of the workgroup share each sequence:
+// they take alternate blocks of 8 k-mers into their own bit-sliced counters,
+// which are added (bit-sliced) before phase 3.
+template <int K, int P>
+__global__ void __launch_bounds__(1024)
+encode_pair_kernel(const uint32_t* __restrict__ gtable, int k_rt, int D, int W32, int TW,
+                   const uint32_t* __restrict__ packed, const uint32_t* __restrict__ invalid,
+                   const int64_t* __restrict__ offsets, const int64_t* __restrict__ starts,
+                   const int32_t* __restrict__ lens, int window_len, int n_seqs,
+                   EncodeOut out) {
+    const int k = K > 0 ? K : k_rt;
+    const int KH = k / 2;
+    extern __shared__ uint32_t smem[];
+    uint32_t* tab2 = smem;                                  // [16][KH][W32]
+    uint32_t* tab1 = tab2 + 16 * KH * W32;                  // [4][W32], odd k: last base
+    uint32_t* codes = tab1 + ((k & 1) ? 4 * W32 : 0);      // [kChunk]
+    uint32_t* xplanes = codes + kChunk;                     // [P][W32]
+    __shared__ int64_t red_sq[16];
+    __shared__ int red_mx[16];
+    __shared__ int red_res[16];
+    __shared__ int s_nvalid;
+
+    const int tid = threadIdx.x;
+    const int nthr = blockDim.x;
+    const int half = tid >= TW ? 1 : 0;
+    const int w = tid - half * TW;
+    const int kW = k * W32;
+    const int KHW = KH * W32;
+    for (int i = tid; i < 16 * KHW; i += nthr) {
+        const int v = i / KHW, rem = i - v * KHW, j = rem / W32, ww = rem - j * W32;
+        tab2[i] = gtable[(v & 3) * kW + (2 * j) * W32 + ww] ^
+                  gtable[(v >> 2) * kW + (2 * j + 1) * W32 + ww];
+    }
+    if (k & 1)
+        for (int i = tid; i < 4 * W32; i += nthr) {
+            const int b = i / W32, ww = i - b * W32;
+            tab1[i] = gtable[b * kW + (k - 1) * W32 + ww];
+        }
+
+    const uint32_t mask2k = (k == 16) ? 0xFFFFFFFFu : ((1u << (2 * k)) - 1u);
+    const uint32_t maskk = (1u << k) - 1u;
+
+    for (int s = blockIdx.x; s < n_seqs; s += gridDim.x) {
+        const int64_t off = starts ? starts[s] : offsets[s];
+        const int64_t len = starts ? (lens ? (int64_t)lens[s] : (int64_t)window_len)
+                                   : offsets[s + 1] - off;
+        const int64_t npos = len - k + 1;
+        uint32_t planes[P];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) { plo[q] = 0; phi[q] = 0; pcl[q] = 0; }
+        for (int p = 0; p < P; ++p) planes[p] = 0;
+        int nvalid = 0;
+
+        for (int64_t c0 = 0; c0 < npos; c0 += kChunk) {
+            const int cn = (int)((npos - c0) < kChunk ? (npos - c0) : kChunk);
+            __syncthreads();
+            for (int j = tid; j < cn; j += nthr) {
+                const int64_t p = off + c0 + j;
+                const uint64_t w2 = (uint64_t)packed[p >> 4] | ((uint64_t)packed[(p >> 4) + 1] << 32);
+                const uint32_t v = (uint32_t)(w2 >> (2 * (p & 15))) & mask2k;
+                const uint64_t iv = (uint64_t)invalid[p >> 5] | ((uint64_t)invalid[(p >> 5) + 1] << 32);
+                const uint32_t bad = (uint32_t)(iv >> (p & 31)) & maskk;
+                uint32_t code = kInvalidCode;
+                if (!bad) {
+                    const uint32_t f = rev_digits(v, k);
+                    const uint32_t r = mask2k - v;
+                    code = (r < f) ? (mask2k - f) : v;
+                }
+                codes[j] = code;
+            }
+            __syncthreads();
+            if (w < W32) {
+                for (int j = 8 * half; j < cn; j += 16) {  // alternate 8-blocks per half
+                    uint32_t x[8];
 #pragma unroll
-            for (int b = 0; b < 32; ++b) {
-                uint32_t pc = 0;
+                    for (int u = 0; u < 8; ++u) {
+                        uint32_t code = (j + u < cn) ? codes[j + u] : kInvalidCode;
+                        code = __builtin_amdgcn_readfirstlane(code);
+                        uint32_t y = 0;
+                        if (code != kInvalidCode) {
+                            ++nvalid;
+                            if (K > 0) {
 #pragma unroll
-                for (int p = 0; p < P; ++p) pc |= ((planes[p] >> b) & 1u) << p;
-                int cnt = (32 * w + b < D) ? (int)(int16_t)(nvalid - 2 * (int)pc) : 0;
-                sq += (int64_t)cnt * cnt;
-                const int a = cnt < 0 ? -cnt : cnt;
-                mx = max(mx, a);
-                res += a > 127 ? a - 127 : 0;
-                const uint32_t u16 = (uint32_t)(uint16_t)(int16_t)cnt;
-                if (b & 1) packed16[b >> 1] |= u16 << 16;
-                else packed16[b >> 1] = u16;
-                const int l8 = (int)(int8_t)(cnt & 0xFF);
-                const int h8 = (cnt - l8) >> 8;
-                const int c8 = cnt > 127 ? 127 : (cnt < -127 ? -127 : cnt);
-                plo[b >> 2] |= ((uint32_t)(uint8_t)l8) << (8 * (b & 3));
-                phi[b >> 2] |= ((uint32_t)(uint8_t)h8) << (8 * (b & 3));
-                pcl[b >> 2] |= ((uint32_t)(uint8_t)c8) << (8 * (b & 3));
-            }
-            const int64_t row = (int64_t)s * ld + 32 * (int64_t)w;
-            if (out.counts) {
-                uint4* dst = reinterpret_cast<uint4*>(out.counts + row);
-#pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    dst[q] = make_uint4(packed16[4 * q], packed16[4 * q + 1], packed16[4 * q + 2],
-                                        packed16[4 * q + 3]);
-            }
-            if (out.lo) {
-                uint4* dst = reinterpret_cast<uint4*>(out.lo + row);
-                dst[0] = make_uint4(plo[0], plo[1], plo[2], plo[3]);
-                dst[1] = make_uint4(plo[4], plo[5], plo[6], plo[7]);
-            }
-            if (out.hi) {
-                uint4* dst = reinterpret_cast<uint4*>(out.hi + row);
-                dst[0] = make_uint4(phi[0], phi[1], phi[2], phi[3]);
-                dst[1] = make_uint4(phi[4], phi[5], phi[6], phi[7]);
-            }
-            if (out.clamp8) {
-                uint4* dst = reinterpret_cast<uint4*>(out.clamp8 + row);
-                dst[0] = make_uint4(pcl[0], pcl[1], pcl[2], pcl[3]);
-                dst[1] = make_uint4(pcl[4], pcl[5], pcl[6], pcl[7]);
+                                for (int jj = 0; jj < (K > 0 ? K / 2 : 1); ++jj)
+                                    y ^= tab2[((code >> (4 * jj)) & 15u) * KHW + jj * W32 + w];
+                                if (K & 1) y ^= tab1[((code >> (2 * (k - 1))) & 3u) * W32 + w];
+                            } else {
+                                for (int jj = 0; jj < KH; ++jj)
+                                    y ^= tab2[((code >> (4 * jj)) & 15u) * KHW + jj * W32 + w];
+                                if (k & 1) y ^= tab1[((code >> (2 * (k - 1))) & 3u) * W32 + w];
+                            }
+                        }
+                        x[u] = y;
+                    }
+                    add8<P>(planes, x);
+                }
             }
         }
-        // zero padding columns beyond the last full word
-        for (int64_t d = 32 * (int64_t)W32 + tid; d < ld; d += nthr) {
-            const int64_t o = (int64_t)s * ld + d;
-            if (out.counts) out.counts[o] = 0;
-            if (out.lo) out.lo[o] = 0;
-            if (out.hi) out.hi[o] = 0;
-            if (out.clamp8) out.clamp8[o] = 0;
-        }
-        sq = wave_sum64(sq);
-        mx = wave_max(mx);
-        res = wave_sum(res);
-        if (lane == 0) { red_sq[wave] = sq; red_mx[wave] = mx; red_res[wave] = res; }
+        // add half 1's counters into half 0's (bit-sliced ripple add)
         __syncthreads();
-        if (tid == 0) {
-            int64_t t = 0;
-            int m = 0, r = 0;
-            for (int q = 0; q < nwaves; ++q) {
-                t += red_sq[q];
-                m = max(m, red_mx[q]);
-                r += red_res[q];
-            }
-            if (out.sqnorm) out.sqnorm[s] = t;
-            if (out.maxabs) out.maxabs[s] = m;
-            if (out.nkmers) out.nkmers[s] = nvalid;
-            if (out.resid1) out.resid1[s] = r;
+        if (half == 1 && w < W32) {
+#pragma unroll
+            for (int p = 0; p < P; ++p) xplanes[p * W32 + w] = planes[p];
+            if (w == 0) s_nvalid = nvalid;
         }
         __syncthreads();
+        if (half == 0 && w < W32) {
+            uint32_t cy = 0;
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                const uint32_t a = planes[p], b = xplanes[p * W32 + w];
+                const uint32_t t = a ^ b;
+                planes[p] = t ^ cy;
+                cy = (a & b) | (t & cy);
+            }
+            nvalid += s_nvalid;
+        }
+        finish_row<P>(planes, half == 0 && w < W32, w, nvalid, D, W32, s, out, red_sq, red_mx,
+                      red_res);
     }
 }
 
@@ -265,13 +404,26 @@ int launch_kp(hvk_ctx* ctx, const uint32_t* packed, const uint32_t* invalid,
               int n, const EncodeOut& out, hipStream_t st) {
     const int W32 = ctx->w32;
     const int threads = (W32 + 63) / 64 * 64;
-    const size_t lds = sizeof(uint32_t) * ((size_t)4 * ctx->k * W32 + kChunk);
-    if (lds + 256 > (size_t)ctx->max_lds) return HVK_ERR_ARG;
+    const int k = ctx->k;
+    // pair table when its 16 x (k/2) rows, two halves of threads and the half
+    // counters fit one workgroup
+    const size_t lds_pair = sizeof(uint32_t) * ((size_t)16 * (k / 2) * W32 +
+                                                ((k & 1) ? 4 * (size_t)W32 : 0) + kChunk +
+                                                (size_t)P * W32);
+    const bool pair = k >= 2 && 2 * threads <= 1024 && lds_pair + 512 <= (size_t)ctx->max_lds;
     int grid = n < 4 * ctx->n_cu ? n : 4 * ctx->n_cu;
     timing_mark(ctx, 0, false, st);
-    hipLaunchKernelGGL((encode_kernel<K, P>), dim3(grid), dim3(threads), lds, st, ctx->d_table,
-                       ctx->k, ctx->dim, W32, packed, invalid, offsets, starts, lens, window_len,
-                       n, out);
+    if (pair) {
+        hipLaunchKernelGGL((encode_pair_kernel<K, P>), dim3(grid), dim3(2 * threads), lds_pair, st,
+                           ctx->d_table, k, ctx->dim, W32, threads, packed, invalid, offsets,
+                           starts, lens, window_len, n, out);
+    } else {
+        const size_t lds = sizeof(uint32_t) * ((size_t)4 * k * W32 + kChunk);
+        if (lds + 256 > (size_t)ctx->max_lds) return HVK_ERR_ARG;
+        hipLaunchKernelGGL((encode_kernel<K, P>), dim3(grid), dim3(threads), lds, st, ctx->d_table,
+                           k, ctx->dim, W32, packed, invalid, offsets, starts, lens, window_len,
+                           n, out);
+    }
     timing_mark(ctx, 0, true, st);
     HVK_HIP(hipGetLastError());
     return HVK_OK;
