@@ -62,6 +62,71 @@ int rescan(hvk_ctx* ctx, const RowsI16& rows, const int32_t* list, const int32_t
     return rescan_impl(ctx, rows, list, count, b, qn, rn, nr, k, idx, score, cosv, n_res, st);
 }
 
+namespace {
+__global__ void order_fill(int32_t* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// best candidate of each query: the largest first key over the split lists
+__global__ void order_keys(const uint64_t* __restrict__ part, int S, int nq, int kp, int nr,
+                           int32_t* __restrict__ okey, int32_t* __restrict__ hist) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nq) return;
+    uint64_t best = 0;
+    for (int s = 0; s < S; ++s) {
+        const uint64_t c = part[((int64_t)s * nq + m) * kp];
+        best = c > best ? c : best;
+    }
+    const int key = best ? idx_of_key(best) : nr;
+    okey[m] = key;
+    atomicAdd(&hist[key], 1);
+}
+
+__global__ void __launch_bounds__(1024)
+order_scan(const int32_t* hist, int n_keys, int32_t* cursor, int32_t* total) {
+    __shared__ int32_t s[1024];
+    const int t = threadIdx.x;
+    const int per = (n_keys + 1023) / 1024;
+    const int a = min(t * per, n_keys), e = min(a + per, n_keys);
+    int sum = 0;
+    for (int i = a; i < e; ++i) sum += hist[i];
+    s[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const int v = t >= off ? s[t - off] : 0;
+        __syncthreads();
+        s[t] += v;
+        __syncthreads();
+    }
+    int run = s[t] - sum;
+    for (int i = a; i < e; ++i) {
+        cursor[i] = run;
+        run += hist[i];
+    }
+    if (t == 1023) *total = s[1023];
+}
+
+__global__ void order_scatter(const int32_t* __restrict__ okey, int nq, int32_t* __restrict__ cursor,
+                              int32_t* __restrict__ order) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= nq) return;
+    order[atomicAdd(&cursor[okey[m]], 1)] = m;
+}
+}  // namespace
+
+int launch_finalize_order(const SearchBufs& b, int S, int nq, int kp, int nr, hipStream_t st) {
+    const int nk = nr + 1;
+    hipLaunchKernelGGL(order_fill, dim3((nk + 255) / 256), dim3(256), 0, st, b.ohist, nk);
+    hipLaunchKernelGGL(order_keys, dim3((nq + 255) / 256), dim3(256), 0, st, b.part, S, nq, kp, nr,
+                       b.okey, b.ohist);
+    hipLaunchKernelGGL(order_scan, dim3(1), dim3(1024), 0, st, b.ohist, nk, b.ocur, b.ototal);
+    hipLaunchKernelGGL(order_scatter, dim3((nq + 255) / 256), dim3(256), 0, st, b.okey, nq, b.ocur,
+                       b.order);
+    HVK_HIP(hipGetLastError());
+    return HVK_OK;
+}
+
 HVK_SEARCH_RUNS(8, extern)
 HVK_SEARCH_RUNS(16, extern)
 HVK_SEARCH_RUNS(32, extern)

@@ -21,6 +21,7 @@
 #pragma once
 
 #include <cmath>
+#include <type_traits>
 #include <utility>
 
 #include "hvk_internal.h"
@@ -865,18 +866,22 @@ finalize_kernel(Rows rows, const uint64_t* __restrict__ part, int n_splits, int 
                 int32_t* __restrict__ out_idx, float* __restrict__ out_score,
                 double* __restrict__ out_cos, int32_t* __restrict__ flag_list,
                 int32_t* __restrict__ flag_count, const int32_t* __restrict__ q_rows,
-                const int32_t* __restrict__ q_count, int part_rows) {
+                const int32_t* __restrict__ q_count, int part_rows,
+                const int32_t* __restrict__ order) {
     // q_rows (nullable): query m is stored row q_rows[m], m < *q_count; the
     // partial lists hold part_rows queries, later ones are forwarded to the
-    // flag list unexamined.
+    // flag list unexamined.  order (nullable): wave i takes query order[i]
+    // (queries grouped by their best candidate, so neighbouring waves re-read
+    // the same reference rows from L2).
     __shared__ uint64_t s_key[4][KP];
     __shared__ double s_c[4][KP];
     __shared__ int s_i[4][KP];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int m = blockIdx.x * 4 + wv;
+    const int i = blockIdx.x * 4 + wv;
     int n_all = nq;
     if (q_rows) n_all = *q_count < nq ? *q_count : nq;
-    if (m >= n_all) return;  // whole wave exits together
+    if (i >= n_all) return;  // whole wave exits together
+    const int m = order ? order[i] : i;
     const int row = q_rows ? q_rows[m] : m;
     if (m >= part_rows) {
         if (lane == 0) flag_list[atomicAdd(flag_count, 1)] = row;
@@ -1018,6 +1023,11 @@ struct SearchBufs {
     float2* br;
     uint64_t* part;
     int32_t* flag_list;   // queued by the first finalize
+    int32_t* okey;        // finalize order: best-candidate index per query
+    int32_t* order;       // queries sorted by okey
+    int32_t* ohist;       // [nr + 1] bucket counts, then cursors
+    int32_t* ocur;
+    int32_t* ototal;
     int32_t* flag_count;
     uint64_t* part2;      // second (exact) pass
     int32_t* flag_list2;  // queued by the second finalize
@@ -1037,7 +1047,7 @@ inline int prepare(hvk_ctx* ctx, int nq, int nr, int n_splits, int kp, SearchBuf
     if (st) return st;
     st = ctx->ws_search.ensure(sizeof(uint64_t) * (size_t)n_splits * nq * kp);
     if (st) return st;
-    st = ctx->ws_flags.ensure(sizeof(int32_t) * (2 * (size_t)nq + 128));
+    st = ctx->ws_flags.ensure(sizeof(int32_t) * (4 * (size_t)nq + 2 * (size_t)nr + 512));
     if (st) return st;
     st = ctx->ws_search2.ensure(sizeof(uint64_t) * (size_t)kPass2Splits * kPass2Cap * kPass2KP);
     if (st) return st;
@@ -1051,6 +1061,11 @@ inline int prepare(hvk_ctx* ctx, int nq, int nr, int n_splits, int kp, SearchBuf
     b.flag_count2 = b.flag_count + 32;
     b.flag_list = b.flag_count + 64;
     b.flag_list2 = b.flag_list + nq + 64;
+    b.okey = b.flag_list2 + nq + 64;
+    b.order = b.okey + nq;
+    b.ohist = b.order + nq;
+    b.ocur = b.ohist + nr + 1;
+    b.ototal = b.ocur + nr + 1;
     b.part2 = (uint64_t*)ctx->ws_search2.ptr;
     return HVK_OK;
 }
@@ -1097,13 +1112,19 @@ inline MfmaPlan plan_mfma(hvk_ctx* ctx, int q_tiles, int r_tiles, int want_split
     return p;
 }
 
+// Orders the queries by the index of their best candidate (a counting sort
+// over the split lists' first keys) into b.order, for finalize.
+int launch_finalize_order(const SearchBufs& b, int S, int nq, int kp, int nr, hipStream_t st);
+
 template <typename Rows, int KP>
 int finalize(hvk_ctx* ctx, const Rows& rows, const SearchBufs& b, int S, const int64_t* qn, int nq,
              const int64_t* rn, int nr, int k, int32_t* idx, float* score, double* cosv,
-             hipStream_t st) {
+             hipStream_t st, bool ordered = false) {
+    if (ordered)
+        if (int rc = launch_finalize_order(b, S, nq, KP, nr, st)) return rc;
     hipLaunchKernelGGL((finalize_kernel<KP, Rows>), dim3((nq + 3) / 4), dim3(256), 0, st, rows,
                        b.part, S, nq, qn, rn, nr, k, idx, score, cosv, b.flag_list, b.flag_count,
-                       nullptr, nullptr, nq);
+                       nullptr, nullptr, nq, ordered ? b.order : nullptr);
     HVK_HIP(hipGetLastError());
     return HVK_OK;
 }
@@ -1132,7 +1153,10 @@ template <typename Rows, int KP>
 int finish(hvk_ctx* ctx, const Rows& rows, const SearchBufs& b, int S, const int64_t* qn, int nq,
            const int64_t* rn, int nr, int k, int32_t* idx, float* score, double* cosv,
            int32_t* n_res, hipStream_t st) {
-    if (int rc = finalize<Rows, KP>(ctx, rows, b, S, qn, nq, rn, nr, k, idx, score, cosv, st))
+    // the limb rows of the MFMA forms are re-read per candidate: order the
+    // queries so neighbouring waves share them in L2
+    if (int rc = finalize<Rows, KP>(ctx, rows, b, S, qn, nq, rn, nr, k, idx, score, cosv, st,
+                                    std::is_same<Rows, RowsLimb>::value))
         return rc;
     return rescan(ctx, rows, b.flag_list, b.flag_count, b, qn, rn, nr, k, idx, score, cosv, n_res,
                   st);
@@ -1200,7 +1224,7 @@ int run_mfma_bounded(hvk_ctx* ctx, const hvk_bundles& q, const hvk_bundles& r, i
     HVK_HIP(hipGetLastError());
     const RowsLimb rows{q.lo, q.hi, r.lo, r.hi, ld};
     if (int rc = finalize<RowsLimb, KP>(ctx, rows, b, pl.map.n_splits, q.sqnorm, nq, r.sqnorm, nr,
-                                        k, idx, score, cosv, st))
+                                        k, idx, score, cosv, st, true))
         return rc;
     return exact_pass2(ctx, q, r, ld, k, b, idx, score, cosv, n_res, st);
 }

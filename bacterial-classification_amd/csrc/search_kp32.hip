@@ -37,7 +37,7 @@ int exact_pass2(hvk_ctx* ctx, const hvk_bundles& q, const hvk_bundles& r, int64_
     HVK_HIP(hipGetLastError());
     hipLaunchKernelGGL((finalize_kernel<kPass2KP, RowsLimb>), dim3((nq + 3) / 4), dim3(256), 0, st,
                        rows, b.part2, S2, nq, q.sqnorm, r.sqnorm, nr, k, idx, score, cosv,
-                       b.flag_list2, b.flag_count2, b.flag_list, b.flag_count, cap);
+                       b.flag_list2, b.flag_count2, b.flag_list, b.flag_count, cap, nullptr);
     HVK_HIP(hipGetLastError());
     return rescan(ctx, rows, b.flag_list2, b.flag_count2, b, q.sqnorm, r.sqnorm, nr, k, idx, score,
                   cosv, n_res, st);
