@@ -32,7 +32,10 @@ EXPORTS = [
     "hvk_descend", "hvk_tree_place", "hvk_tree_sum", "hvk_tree_limbs",
     "hvk_batch_create", "hvk_batch_destroy", "hvk_batch_config", "hvk_batch_get_config",
     "hvk_batch_classify", "hvk_batch_autotune",
+    "hvk_db_write", "hvk_db_open", "hvk_db_close", "hvk_db_info", "hvk_db_array", "hvk_db_upload",
 ]
+
+DB_COUNTS, DB_LO, DB_HI, DB_CLAMP8, DB_SUMS = 1, 2, 4, 8, 16
 
 BATCH_FORM_BOUNDED = 0
 BATCH_FORM_TWO_LIMB = 1
@@ -130,6 +133,16 @@ def lib():
     L.hvk_tree_place.argtypes = [vp, vp, i64, i32, vp, vp, i64, vp]
     L.hvk_tree_sum.argtypes = [vp, vp, i64, i32, vp, vp, vp, i32, vp, vp]
     L.hvk_tree_limbs.argtypes = [vp, vp, i32, i64, vp, vp, vp, vp, vp]
+    L.hvk_db_write.argtypes = [ctypes.c_char_p, i32, i32, i64, i32, ctypes.c_uint32,
+                               ctypes.POINTER(EncodeOut), vp, i64]
+    L.hvk_db_open.argtypes = [ctypes.c_char_p, ctypes.POINTER(vp)]
+    L.hvk_db_close.argtypes = [vp]
+    L.hvk_db_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                              ctypes.POINTER(i64), ctypes.POINTER(ctypes.c_int),
+                              ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(i64)]
+    L.hvk_db_array.restype = vp
+    L.hvk_db_array.argtypes = [vp, i32, ctypes.POINTER(i64)]
+    L.hvk_db_upload.argtypes = [vp, vp, ctypes.POINTER(EncodeOut)]
     L.hvk_batch_create.argtypes = [i32, i32, i32, ctypes.POINTER(vp)]
     L.hvk_batch_destroy.argtypes = [vp]
     L.hvk_batch_config.argtypes = [vp, i32, i32, i32]
@@ -629,3 +642,98 @@ class Batch:
                                        _ptr(out.idx), _ptr(out.score), sample, ctypes.byref(ms),
                                        _stream()), "hvk_batch_autotune")
         return ms.value
+
+
+# ----------------------------------------------------------------------------
+# prototype database file
+# ----------------------------------------------------------------------------
+def db_write_host(path, dim, kmer, ld, arrays: dict, names: bytes | None = None):
+    """Writes a database from host numpy arrays: keys counts (int16 [n][ld]),
+    lo, hi, clamp8 (int8 [n][ld]), and sqnorm, maxabs, resid1, nkmers (all
+    four or none)."""
+    flags = 0
+    o = EncodeOut()
+    keep = []
+    n = None
+    for key, flag, dt in (("counts", DB_COUNTS, np.int16), ("lo", DB_LO, np.int8),
+                          ("hi", DB_HI, np.int8), ("clamp8", DB_CLAMP8, np.int8)):
+        if arrays.get(key) is not None:
+            a = np.ascontiguousarray(arrays[key], dt)
+            keep.append(a)
+            setattr(o, key, a.ctypes.data)
+            flags |= flag
+            n = a.shape[0]
+    if arrays.get("sqnorm") is not None:
+        for key, dt in (("sqnorm", np.int64), ("maxabs", np.int32), ("resid1", np.int32),
+                        ("nkmers", np.int32)):
+            a = np.ascontiguousarray(arrays[key], dt)
+            keep.append(a)
+            setattr(o, key, a.ctypes.data)
+            n = a.shape[0]
+        flags |= DB_SUMS
+    rows = [x for x in keep if x.ndim == 2]
+    if any(x.shape[1] != rows[0].shape[1] for x in rows):
+        raise ValueError("row arrays must share one stride")
+    o.ld = rows[0].shape[1] if rows else ld  # source stride; the file's is ld
+    nb = names or b""
+    check(lib().hvk_db_write(path.encode(), dim, kmer, ld, n or 0, flags, ctypes.byref(o),
+                             nb if nb else None, len(nb)), "hvk_db_write")
+
+
+def db_save(path, ctx: Context, e: Encoded, names: bytes | None = None):
+    """Writes an encoded database (device arrays of e) to `path`."""
+    arrays = {k: (getattr(e, k).cpu().numpy() if getattr(e, k) is not None else None)
+              for k in ("counts", "lo", "hi", "clamp8")}
+    if e.resid1 is not None:
+        arrays.update(sqnorm=e.sqnorm.cpu().numpy(), maxabs=e.maxabs.cpu().numpy(),
+                      resid1=e.resid1.cpu().numpy(), nkmers=e.nkmers.cpu().numpy())
+    db_write_host(path, ctx.dim, ctx.kmer, ctx.ld, arrays, names)
+
+
+class DbFile:
+    """A mapped database file (hvk_db_open)."""
+
+    def __init__(self, path):
+        h = ctypes.c_void_p()
+        check(lib().hvk_db_open(path.encode(), ctypes.byref(h)), "hvk_db_open")
+        self.h = h
+        d, k, n = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        ld, nl = ctypes.c_int64(), ctypes.c_int64()
+        fl = ctypes.c_uint32()
+        check(lib().hvk_db_info(h, ctypes.byref(d), ctypes.byref(k), ctypes.byref(ld),
+                                ctypes.byref(n), ctypes.byref(fl), ctypes.byref(nl)), "hvk_db_info")
+        self.dim, self.kmer, self.ld, self.n = d.value, k.value, ld.value, n.value
+        self.flags, self.names_len = fl.value, nl.value
+
+    def array(self, which, dtype, shape):
+        b = ctypes.c_int64()
+        p = lib().hvk_db_array(self.h, which, ctypes.byref(b))
+        if not p:
+            return None
+        buf = (ctypes.c_char * b.value).from_address(p)
+        return np.frombuffer(buf, dtype=dtype).reshape(shape)
+
+    def names(self) -> bytes:
+        b = ctypes.c_int64()
+        p = lib().hvk_db_array(self.h, 64, ctypes.byref(b))
+        return ctypes.string_at(p, b.value) if p else b""
+
+    def load(self, ctx: Context) -> Encoded:
+        """Device copy of the arrays (row stride ctx.ld)."""
+        f = self.flags
+        e = _alloc(self.n, ctx.ld, counts=bool(f & DB_COUNTS), limbs=bool(f & DB_LO),
+                   hi=bool(f & DB_HI), clamp=bool(f & DB_CLAMP8))
+        o = _out_struct(e, ctx.ld)
+        check(lib().hvk_db_upload(self.h, ctx.h, ctypes.byref(o)), "hvk_db_upload")
+        return e
+
+    def close(self):
+        if self.h:
+            lib().hvk_db_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -277,6 +277,44 @@ int hvk_tree_sum(hvk_ctx* ctx, int32_t* d_proto, int64_t ld, int n_cat, const in
 int hvk_tree_limbs(hvk_ctx* ctx, const int32_t* d_proto, int n, int64_t ld, int16_t* d_lo,
                    int16_t* d_hi, int64_t* d_sqnorm, int32_t* d_maxabs_limb, hvk_stream stream);
 
+/* ---- prototype database file ---------------------------------------------
+ * A versioned binary file of encoded reference prototypes, so references are
+ * encoded once (no counterpart in the reference, which re-encodes its FASTA
+ * files on every run through HVCache, src/hierarchy.cpp:56-81 and
+ * src/encoder.cpp:76-104).  Layout, little-endian:
+ *   128-byte header: magic "HVKPDB01", version (1), header size, D, k, row
+ *   stride ld, row count n, array flags, names size, FNV-1a 64 of the first
+ *   120 header bytes;
+ *   then, each 4096-byte aligned, the arrays the flags name, in this order:
+ *   counts int16 [n][ld], lo int8 [n][ld], hi int8 [n][ld], clamp8 int8
+ *   [n][ld], sqnorm int64 [n], maxabs int32 [n], resid1 int32 [n],
+ *   nkmers int32 [n], then the names ('\n'-separated, optional).
+ * hvk_db_open maps the file read-only and validates magic, version, header
+ * checksum and size; the arrays are then readable in place (host pointers
+ * into the mapping) or copied to device buffers with hvk_db_upload. */
+#define HVK_DB_COUNTS 1u
+#define HVK_DB_LO     2u
+#define HVK_DB_HI     4u
+#define HVK_DB_CLAMP8 8u
+#define HVK_DB_SUMS   16u  /* sqnorm, maxabs, resid1, nkmers */
+typedef struct hvk_db hvk_db;
+/* Writes a database from host arrays (`arrays` in host memory, ld = its row
+ * stride; arrays not named by `flags` may be NULL).  names: optional blob of
+ * names_len bytes. */
+int hvk_db_write(const char* path, int dim, int kmer, int64_t ld, int n, uint32_t flags,
+                 const hvk_encode_out* arrays, const char* names, int64_t names_len);
+int hvk_db_open(const char* path, hvk_db** out);
+int hvk_db_close(hvk_db* db);
+int hvk_db_info(const hvk_db* db, int* dim, int* kmer, int64_t* ld, int* n, uint32_t* flags,
+                int64_t* names_len);
+/* Host view of one array (HVK_DB_COUNTS ... HVK_DB_CLAMP8, or 32 + i for the
+ * row sums sqnorm, maxabs, resid1, nkmers; 64 for the names); NULL when absent. */
+const void* hvk_db_array(const hvk_db* db, int which, int64_t* bytes);
+/* Copies the arrays present in both the file and `dst` (device buffers, row
+ * stride dst->ld or the file's ld when 0) to the device; HVK_ERR_ARG when
+ * ctx's D, k differ from the file's.  Synchronous. */
+int hvk_db_upload(const hvk_db* db, const hvk_ctx* ctx, const hvk_encode_out* dst);
+
 /* ---- batching form -------------------------------------------------------
  * No counterpart in the reference, whose reads go one at a time through
  * HVCache::get and cosine (targets/identify.cpp:150-157).  Classifies a large
