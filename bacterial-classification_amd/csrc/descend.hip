@@ -596,7 +596,7 @@ extern "C" int hvk_descend(hvk_ctx* ctx, const hvk_tree* tree, const int16_t* d_
 }
 
 extern "C" int hvk_tree_place(hvk_ctx* ctx, const int16_t* d_counts, int64_t counts_ld, int n,
-                              const int32_t* h_cat, int32_t* d_proto, int64_t ld,
+                              const int32_t* h_cat, int32_t* d_proto, int64_t ld, int n_cat,
                               hvk_stream stream) {
     if (!ctx || n < 0) return HVK_ERR_ARG;
     if (n == 0) return HVK_OK;
@@ -605,7 +605,7 @@ extern "C" int hvk_tree_place(hvk_ctx* ctx, const int16_t* d_counts, int64_t cou
     if (ld == 0) ld = ctx->ld;
     if (counts_ld < ctx->dim || ld < ctx->dim) return HVK_ERR_ARG;
     for (int i = 0; i < n; ++i)
-        if (h_cat[i] < 0) return HVK_ERR_ARG;
+        if (h_cat[i] < 0 || h_cat[i] >= n_cat) return HVK_ERR_ARG;
     std::lock_guard<std::mutex> g(ctx->mu);
     hipStream_t st = (hipStream_t)stream;
     std::vector<const int32_t*> dev;

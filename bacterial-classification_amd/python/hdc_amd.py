@@ -130,7 +130,7 @@ def lib():
     L.hvk_gen_fill.argtypes = [ctypes.POINTER(GenSpec), vp, vp, vp, vp, vp]
     L.hvk_descend.argtypes = [vp, ctypes.POINTER(Tree), vp, vp, i32, i64, i32, i32,
                               ctypes.c_float, i32, vp, vp, vp, vp, vp]
-    L.hvk_tree_place.argtypes = [vp, vp, i64, i32, vp, vp, i64, vp]
+    L.hvk_tree_place.argtypes = [vp, vp, i64, i32, vp, vp, i64, i32, vp]
     L.hvk_tree_sum.argtypes = [vp, vp, i64, i32, vp, vp, vp, i32, vp, vp]
     L.hvk_tree_limbs.argtypes = [vp, vp, i32, i64, vp, vp, vp, vp, vp]
     L.hvk_db_write.argtypes = [ctypes.c_char_p, i32, i32, i64, i32, ctypes.c_uint32,
@@ -549,7 +549,7 @@ def build_prototypes(ctx: Context, tax: Taxonomy, leaf_counts, leaf_cat,
     proto = torch.zeros((n_cat, ld), dtype=torch.int32, device="cuda")
     lc = np.ascontiguousarray(leaf_cat, np.int32)
     check(L.hvk_tree_place(ctx.h, _ptr(leaf_counts), ld, len(lc), lc.ctypes.data, _ptr(proto), ld,
-                           _stream()), "hvk_tree_place")
+                           n_cat, _stream()), "hvk_tree_place")
     ovf = torch.zeros(1, dtype=torch.int32, device="cuda")
     span = np.diff(tax.child_off)
     self_list = (span == 1) & (tax.child_idx[tax.child_off[:-1]] == np.arange(n_cat))

@@ -270,7 +270,8 @@ int hvk_descend(hvk_ctx* ctx, const hvk_tree* tree, const int16_t* d_q_counts,
  * (nullable) to 1.  hvk_tree_limbs splits int32 rows into the lo/hi limbs
  * and exact squared norms hvk_descend reads, and each row's max |limb|. */
 int hvk_tree_place(hvk_ctx* ctx, const int16_t* d_counts, int64_t counts_ld, int n,
-                   const int32_t* h_cat, int32_t* d_proto, int64_t ld, hvk_stream stream);
+                   const int32_t* h_cat, int32_t* d_proto, int64_t ld, int n_cat,
+                   hvk_stream stream);
 int hvk_tree_sum(hvk_ctx* ctx, int32_t* d_proto, int64_t ld, int n_cat, const int32_t* h_child_off,
                  const int32_t* h_child_idx, const int32_t* h_parents, int n_parents,
                  int32_t* d_overflow, hvk_stream stream);
