@@ -32,6 +32,11 @@ constexpr int kDescG = 8;  // candidates scored per pass over a read row
 constexpr int kDescThreads = 256;
 constexpr int kMaxBeam = 16;
 constexpr int kMaxLevels = 32;
+// bucket counters are kept in kDescCopies copies per key (workgroup b uses copy
+// b % kDescCopies) so that few hot keys -- 40 phyla for 3M pairs at level 1 --
+// do not serialise on one atomic; the scan runs over (key, copy) in key-major
+// order, so the pairs stay grouped by key
+constexpr int kDescCopies = 8;
 
 typedef short short2_t __attribute__((ext_vector_type(2)));
 
@@ -119,7 +124,7 @@ __global__ void desc_hist(Beams b, int n, int B, int n_cat, int32_t* hist) {
             const int c = b.cat[(int64_t)r * B + j];
             key = c < 0 ? n_cat : c;
         }
-        (void)wave_agg_add(hist, key, act);
+        (void)wave_agg_add(hist, key * kDescCopies + (int)(blockIdx.x % kDescCopies), act);
     }
 }
 
@@ -158,7 +163,7 @@ __global__ void desc_scatter(Beams b, int n, int B, int n_cat, int32_t* cursor, 
             const int c = b.cat[(int64_t)r * B + j];
             key = c < 0 ? n_cat : c;
         }
-        const int pos = wave_agg_add(cursor, key, act);
+        const int pos = wave_agg_add(cursor, key * kDescCopies + (int)(blockIdx.x % kDescCopies), act);
         if (act) {
             pairs[pos] = r * B + j;
             pkey[pos] = key;
@@ -538,7 +543,7 @@ extern "C" int hvk_descend(hvk_ctx* ctx, const hvk_tree* tree, const int16_t* d_
     const size_t beam_bytes = align_up(n * 4) + align_up(n * B * 4) + 2 * align_up(n * B * L * 4);
     const size_t sc_bytes = align_up((size_t)n * B * Fmax * 4);
     const size_t pair_bytes = 2 * align_up((size_t)n * B * 4);
-    const size_t key_bytes = 2 * align_up(((size_t)n_cat + 1) * 4) + 256;
+    const size_t key_bytes = 2 * align_up(((size_t)n_cat + 1) * kDescCopies * 4) + 256;
     const size_t sort_bytes = 2 * align_up((size_t)n * maxN * 4);
     if (int e = ctx->ws_desc.ensure(2 * beam_bytes + sc_bytes + pair_bytes + key_bytes + sort_bytes))
         return e;
@@ -554,8 +559,8 @@ extern "C" int hvk_descend(hvk_ctx* ctx, const hvk_tree* tree, const int16_t* d_
     float* sc = (float*)take((size_t)n * B * Fmax * 4);
     int32_t* pairs = (int32_t*)take((size_t)n * B * 4);
     int32_t* pkey = (int32_t*)take((size_t)n * B * 4);
-    int32_t* hist = (int32_t*)take(((size_t)n_cat + 1) * 4);
-    int32_t* cursor = (int32_t*)take(((size_t)n_cat + 1) * 4);
+    int32_t* hist = (int32_t*)take(((size_t)n_cat + 1) * kDescCopies * 4);
+    int32_t* cursor = (int32_t*)take(((size_t)n_cat + 1) * kDescCopies * 4);
     int32_t* total = (int32_t*)take(256);
     float* ws_s = (float*)take((size_t)n * maxN * 4);
     int32_t* ws_c = (int32_t*)take((size_t)n * maxN * 4);
@@ -563,7 +568,7 @@ extern "C" int hvk_descend(hvk_ctx* ctx, const hvk_tree* tree, const int16_t* d_
     const DescOut out{d_path, d_path_score, d_path_len, d_exit_level};
     const int rb = (int)((n + kDescThreads - 1) / kDescThreads);
     const int pb = (int)((n * B + kDescP * (kDescThreads / 64) - 1) / (kDescP * (kDescThreads / 64)));
-    const int n_keys = n_cat + 1;
+    const int n_keys = (n_cat + 1) * kDescCopies;  // counter slots
     const int64_t K = std::min(q_ld, tld);
 
     timing_mark(ctx, 1, false, st);
