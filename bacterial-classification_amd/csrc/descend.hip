@@ -307,7 +307,10 @@ struct DescOut {
 __global__ void desc_select(TreeDev t, Beams cur, Beams nxt, int n, int B, int L, int lvl,
                             float thr, const float* __restrict__ sc, int Fmax,
                             float* __restrict__ ws_s, int32_t* __restrict__ ws_c, int maxN,
-                            DescOut out) {
+                            DescOut out, int in_lds) {
+    // candidate lists: columns of LDS (element i of thread t at [i][t]) when
+    // in_lds, else a row of the global workspace
+    extern __shared__ float sel_lds[];
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
     const int nb = cur.nb[r];
@@ -315,8 +318,10 @@ __global__ void desc_select(TreeDev t, Beams cur, Beams nxt, int n, int B, int L
         nxt.nb[r] = 0;
         return;
     }
-    float* s = ws_s + (int64_t)r * maxN;
-    int32_t* c = ws_c + (int64_t)r * maxN;
+    float* s = in_lds ? sel_lds + threadIdx.x : ws_s + (int64_t)r * maxN;
+    int32_t* c = in_lds ? reinterpret_cast<int32_t*>(sel_lds + (size_t)maxN * blockDim.x) + threadIdx.x
+                        : ws_c + (int64_t)r * maxN;
+    const int st = in_lds ? (int)blockDim.x : 1;
     int m = 0;
     for (int j = 0; j < nb; ++j) {
         const int32_t* list;
@@ -324,12 +329,12 @@ __global__ void desc_select(TreeDev t, Beams cur, Beams nxt, int n, int B, int L
         span_of(t, cur.cat[(int64_t)r * B + j], list, F);
         const float* row = sc + ((int64_t)r * B + j) * Fmax;
         for (int f = 0; f < F; ++f) {
-            s[m] = row[f];
-            c[m] = j * Fmax + f;
+            s[m * st] = row[f];
+            c[m * st] = j * Fmax + f;
             ++m;
         }
     }
-    lsort::sort_desc(s, c, m);
+    lsort::sort_desc(s, c, m, st);
     // candidate id -> (beam entry, category)
     auto resolve = [&](int id, int& j, int& cat) {
         j = id / Fmax;
@@ -358,10 +363,10 @@ __global__ void desc_select(TreeDev t, Beams cur, Beams nxt, int n, int B, int L
         return;
     }
     int keep = 0;
-    while (keep < m && keep < B && s[keep] >= thr) ++keep;
+    while (keep < m && keep < B && s[keep * st] >= thr) ++keep;
     for (int i = 0; i < keep; ++i) {
         int j, cat;
-        resolve(c[i], j, cat);
+        resolve(c[i * st], j, cat);
         const int64_t src = ((int64_t)r * B + j) * L;
         const int64_t dst = ((int64_t)r * B + i) * L;
         for (int x = 0; x < lvl; ++x) {
@@ -369,7 +374,7 @@ __global__ void desc_select(TreeDev t, Beams cur, Beams nxt, int n, int B, int L
             nxt.pscore[dst + x] = cur.pscore[src + x];
         }
         nxt.path[dst + lvl] = cat;
-        nxt.pscore[dst + lvl] = s[i];
+        nxt.pscore[dst + lvl] = s[i * st];
         nxt.cat[(int64_t)r * B + i] = cat;
     }
     nxt.nb[r] = keep;
@@ -571,6 +576,10 @@ extern "C" int hvk_descend(hvk_ctx* ctx, const hvk_tree* tree, const int16_t* d_
     const int n_keys = (n_cat + 1) * kDescCopies;  // counter slots
     const int64_t K = std::min(q_ld, tld);
 
+    // candidate lists of up to kSelMaxLds entries are sorted in LDS
+    constexpr int kSelThreads = 128, kSelMaxLds = 64;
+    const bool sel_lds = maxN <= kSelMaxLds;
+    const size_t sel_bytes = sel_lds ? 2 * sizeof(float) * (size_t)maxN * kSelThreads : 0;
     timing_mark(ctx, 1, false, st);
     hipLaunchKernelGGL(desc_init, dim3(rb), dim3(kDescThreads), 0, st, bm[0], n_q, B);
     HVK_HIP(hipGetLastError());
@@ -589,8 +598,14 @@ extern "C" int hvk_descend(hvk_ctx* ctx, const hvk_tree* tree, const int16_t* d_
         else
             hipLaunchKernelGGL(desc_score<2>, dim3(pb), dim3(kDescThreads), 0, st, t, d_q_counts,
                                q_ld, d_q_sqnorm, B, K, pairs, pkey, total, Fmax, sc);
-        hipLaunchKernelGGL(desc_select, dim3(rb), dim3(kDescThreads), 0, st, t, bm[cur],
-                           bm[cur ^ 1], n_q, B, L, lvl, threshold, sc, Fmax, ws_s, ws_c, maxN, out);
+        if (sel_lds)
+            hipLaunchKernelGGL(desc_select, dim3((n_q + kSelThreads - 1) / kSelThreads),
+                               dim3(kSelThreads), sel_bytes, st, t, bm[cur], bm[cur ^ 1], n_q, B, L,
+                               lvl, threshold, sc, Fmax, ws_s, ws_c, maxN, out, 1);
+        else
+            hipLaunchKernelGGL(desc_select, dim3(rb), dim3(kDescThreads), 0, st, t, bm[cur],
+                               bm[cur ^ 1], n_q, B, L, lvl, threshold, sc, Fmax, ws_s, ws_c, maxN,
+                               out, 0);
         HVK_HIP(hipGetLastError());
         cur ^= 1;
     }

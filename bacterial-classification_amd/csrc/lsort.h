@@ -28,15 +28,18 @@ namespace lsort {
 struct Pairs {
     float* s;
     int* c;
-    HVK_HD bool less(int a, int b) const { return s[a] > s[b]; }   // comp(*a, *b)
-    HVK_HD bool less_v(float v, int b) const { return v > s[b]; }  // comp(val, *b)
-    HVK_HD bool less_to_v(int a, float v) const { return s[a] > v; }  // comp(*a, val)
+    int st;  // element a lives at s[a * st], c[a * st]
+    HVK_HD float& S(int a) const { return s[a * st]; }
+    HVK_HD int& C(int a) const { return c[a * st]; }
+    HVK_HD bool less(int a, int b) const { return S(a) > S(b); }   // comp(*a, *b)
+    HVK_HD bool less_v(float v, int b) const { return v > S(b); }  // comp(val, *b)
+    HVK_HD bool less_to_v(int a, float v) const { return S(a) > v; }  // comp(*a, val)
     HVK_HD void swap(int a, int b) const {
-        const float ts = s[a]; s[a] = s[b]; s[b] = ts;
-        const int tc = c[a]; c[a] = c[b]; c[b] = tc;
+        const float ts = S(a); S(a) = S(b); S(b) = ts;
+        const int tc = C(a); C(a) = C(b); C(b) = tc;
     }
-    HVK_HD void move(int dst, int src) const { s[dst] = s[src]; c[dst] = c[src]; }
-    HVK_HD void put(int dst, float vs, int vc) const { s[dst] = vs; c[dst] = vc; }
+    HVK_HD void move(int dst, int src) const { S(dst) = S(src); C(dst) = C(src); }
+    HVK_HD void put(int dst, float vs, int vc) const { S(dst) = vs; C(dst) = vc; }
 };
 
 constexpr int kThreshold = 16;
@@ -50,8 +53,8 @@ HVK_HD int floor_log2(int n) {
 // insertion of element `last` into the sorted run ending before it; no lower
 // bound check (an element not greater than it is known to exist before it)
 HVK_HD void unguarded_linear_insert(const Pairs& p, int last) {
-    const float vs = p.s[last];
-    const int vc = p.c[last];
+    const float vs = p.S(last);
+    const int vc = p.C(last);
     int next = last - 1;
     while (p.less_v(vs, next)) {
         p.move(last, next);
@@ -65,8 +68,8 @@ HVK_HD void insertion_sort(const Pairs& p, int first, int last) {
     if (first == last) return;
     for (int i = first + 1; i < last; ++i) {
         if (p.less(i, first)) {
-            const float vs = p.s[i];
-            const int vc = p.c[i];
+            const float vs = p.S(i);
+            const int vc = p.C(i);
             for (int j = i; j > first; --j) p.move(j, j - 1);
             p.put(first, vs, vc);
         } else {
@@ -148,15 +151,15 @@ HVK_HD void heap_sort(const Pairs& p, int first, int last) {
     if (len >= 2) {  // make heap
         int parent = (len - 2) / 2;
         while (true) {
-            adjust_heap(p, first, parent, len, p.s[first + parent], p.c[first + parent]);
+            adjust_heap(p, first, parent, len, p.S(first + parent), p.C(first + parent));
             if (parent == 0) break;
             --parent;
         }
     }
     while (last - first > 1) {  // pop the top to the back
         --last;
-        const float vs = p.s[last];
-        const int vc = p.c[last];
+        const float vs = p.S(last);
+        const int vc = p.C(last);
         p.move(last, first);
         adjust_heap(p, first, 0, last - first, vs, vc);
     }
@@ -187,10 +190,11 @@ HVK_HD void introsort_loop(const Pairs& p, int first, int last) {
     }
 }
 
-// std::sort(s, s + n) with comp(a, b) = a > b, carrying c along
-HVK_HD void sort_desc(float* s, int* c, int n) {
+// std::sort(s, s + n) with comp(a, b) = a > b, carrying c along; element i
+// at s[i * stride], c[i * stride]
+HVK_HD void sort_desc(float* s, int* c, int n, int stride = 1) {
     if (n <= 1) return;
-    const Pairs p{s, c};
+    const Pairs p{s, c, stride};
     introsort_loop(p, 0, n);
     final_insertion_sort(p, 0, n);
 }

@@ -26,17 +26,27 @@ static bool check(const std::vector<float>& s, long& cases) {
     std::vector<Cand> ref(n);
     for (int i = 0; i < n; ++i) ref[i] = {s[i], i};
     std::sort(ref.begin(), ref.end(), [](const Cand& a, const Cand& b) { return a.score > b.score; });
-    std::vector<float> ms(s);
-    std::vector<int> mc(n);
-    for (int i = 0; i < n; ++i) mc[i] = i;
-    hvk::lsort::sort_desc(ms.data(), mc.data(), n);
-    ++cases;
-    for (int i = 0; i < n; ++i)
-        if (ref[i].id != mc[i] || ref[i].score != ms[i]) {
-            std::printf("mismatch n=%d at %d: std %d (%g) mine %d (%g)\n", n, i, ref[i].id,
-                        ref[i].score, mc[i], ms[i]);
-            return false;
+    for (int stride : {1, 3}) {  // contiguous, and interleaved as in LDS columns
+        std::vector<float> ms((size_t)n * stride + 1, -9.0f);
+        std::vector<int> mc((size_t)n * stride + 1, -9);
+        for (int i = 0; i < n; ++i) {
+            ms[(size_t)i * stride] = s[i];
+            mc[(size_t)i * stride] = i;
         }
+        hvk::lsort::sort_desc(ms.data(), mc.data(), n, stride);
+        ++cases;
+        for (int i = 0; i < n; ++i)
+            if (ref[i].id != mc[(size_t)i * stride] || ref[i].score != ms[(size_t)i * stride]) {
+                std::printf("mismatch n=%d stride=%d at %d: std %d (%g) mine %d (%g)\n", n, stride,
+                            i, ref[i].id, ref[i].score, mc[(size_t)i * stride],
+                            ms[(size_t)i * stride]);
+                return false;
+            }
+        for (int i = 0; i < n; ++i)  // the gaps are untouched
+            for (int g = 1; g < stride; ++g)
+                if (ms[(size_t)i * stride + g] != -9.0f || mc[(size_t)i * stride + g] != -9)
+                    return false;
+    }
     return true;
 }
 

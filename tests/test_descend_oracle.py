@@ -30,7 +30,7 @@ def test_lsort_matches_std_sort(tmp_path):
     print(p.stdout)
     assert p.returncode == 0, p.stdout
     cases, heap = p.stdout.split()[1], p.stdout.split()[2]
-    assert int(cases) > 200000 and int(heap.split("=")[1]) > 0
+    assert int(cases) > 400000 and int(heap.split("=")[1]) > 0
 
 
 def load_tax_short():
