@@ -297,6 +297,122 @@ desc_score(TreeDev t, const int16_t* __restrict__ q, int64_t qld, const int64_t*
     }
 }
 
+// Level 0 as a limb GEMM on the matrix cores.  At level 0 every read scores
+// the same candidate list (the roots), so the scores are a dense product
+// reads x roots x D.  Operands are split into signed base-256 digits: a read
+// count q = q0 + 256 q1 (|q| <= 32639), a prototype entry
+// p = d0 + 256 d1 + 2^16 d2 + 2^24 d3 (|p| < 2^30), all in [-128, 127].
+// v_mfma_i32_32x32x32_i8 multiplies digit pairs; the 8 products fall into 5
+// weights 2^(8w) (w = a + b), each accumulated in int32 -- at most 2 products
+// x 10240 x 128 x 128 < 2^31 -- and combined in int64 at the end: the exact
+// dot.  A wave owns 32 reads; the workgroup (4 waves) stages each 32-element
+// K step of the roots' digits in LDS once.  Lane l holds row / column l % 32
+// and K bytes [16 (l / 32), +16) of both operands, as the MFMA pairs them.
+typedef int v4i_t __attribute__((ext_vector_type(4)));
+typedef int v16i_t __attribute__((ext_vector_type(16)));
+constexpr int kDenseN = 64;     // roots (two 32-wide blocks)
+constexpr int kDenseRow = 48;   // LDS bytes per (digit, root) row: 32 + 16 pad, conflict-free
+
+__device__ __forceinline__ uint32_t pack4(int a, int b, int c, int d) {
+    return (uint32_t)(a & 0xFF) | ((uint32_t)(b & 0xFF) << 8) | ((uint32_t)(c & 0xFF) << 16) |
+           ((uint32_t)(d & 0xFF) << 24);
+}
+
+__global__ void __launch_bounds__(256)
+desc_dense(TreeDev t, const int16_t* __restrict__ q, int64_t qld, const int64_t* __restrict__ qn,
+           int n, int B, int64_t K, int Fmax, float* __restrict__ sc) {
+    __shared__ __attribute__((aligned(16))) int8_t bl[4][kDenseN][kDenseRow];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int r0 = (blockIdx.x * 4 + wave) * 32;
+    const int i = lane & 31, kg = lane >> 5;
+    const int16_t* __restrict__ qrow = q + (int64_t)min(r0 + i, n - 1) * qld;  // rows past n: clamped
+    // staging: thread -> (root bc, 8 elements at bk) of the K step
+    const int bc = tid >> 2, bk = (tid & 3) * 8;
+    const int catB = bc < t.n_roots ? t.roots[bc] : -1;
+    const int16_t* __restrict__ blo = t.lo + (int64_t)(catB < 0 ? 0 : catB) * t.ld;
+    const int16_t* __restrict__ bhi = t.hi + (int64_t)(catB < 0 ? 0 : catB) * t.ld;
+    v16i_t acc[5][2];
+#pragma unroll
+    for (int w = 0; w < 5; ++w)
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb) acc[w][nb] = v16i_t{};
+    for (int64_t k0 = 0; k0 < K; k0 += 32) {
+        __syncthreads();  // the previous step's digits are consumed
+        {
+            int d[4][8];
+            if (catB >= 0) {
+                const int4 lv = *reinterpret_cast<const int4*>(blo + k0 + bk);
+                const int4 hv = *reinterpret_cast<const int4*>(bhi + k0 + bk);
+                const int lw[4] = {lv.x, lv.y, lv.z, lv.w}, hw[4] = {hv.x, hv.y, hv.z, hv.w};
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int lo = (int)(int16_t)(lw[e >> 1] >> (16 * (e & 1)));
+                    const int hi = (int)(int16_t)(hw[e >> 1] >> (16 * (e & 1)));
+                    int p = lo + hi * 65536;
+                    d[0][e] = (int)(int8_t)p;
+                    p = (p - d[0][e]) >> 8;
+                    d[1][e] = (int)(int8_t)p;
+                    p = (p - d[1][e]) >> 8;
+                    d[2][e] = (int)(int8_t)p;
+                    d[3][e] = (p - d[2][e]) >> 8;
+                }
+            } else {
+#pragma unroll
+                for (int g = 0; g < 4; ++g)
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) d[g][e] = 0;
+            }
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+                *reinterpret_cast<uint2*>(&bl[g][bc][bk]) =
+                    make_uint2(pack4(d[g][0], d[g][1], d[g][2], d[g][3]),
+                               pack4(d[g][4], d[g][5], d[g][6], d[g][7]));
+        }
+        // the read's digits for K bytes [16 kg, +16) of this step
+        const int4 qa = *reinterpret_cast<const int4*>(qrow + k0 + 16 * kg);
+        const int4 qb = *reinterpret_cast<const int4*>(qrow + k0 + 16 * kg + 8);
+        const int qw[8] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w};
+        int q0[16], q1[16];
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            const int v = (int)(int16_t)(qw[e >> 1] >> (16 * (e & 1)));
+            q0[e] = (int)(int8_t)v;
+            q1[e] = (v - q0[e]) >> 8;
+        }
+        v4i_t a[2];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            a[0][c] = (int)pack4(q0[4 * c], q0[4 * c + 1], q0[4 * c + 2], q0[4 * c + 3]);
+            a[1][c] = (int)pack4(q1[4 * c], q1[4 * c + 1], q1[4 * c + 2], q1[4 * c + 3]);
+        }
+        __syncthreads();  // the step's digits are staged
+#pragma unroll
+        for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const v4i_t b = *reinterpret_cast<const v4i_t*>(&bl[g][nb * 32 + i][16 * kg]);
+#pragma unroll
+                for (int qa_i = 0; qa_i < 2; ++qa_i)
+                    acc[qa_i + g][nb] =
+                        __builtin_amdgcn_mfma_i32_32x32x32_i8(a[qa_i], b, acc[qa_i + g][nb], 0, 0, 0);
+            }
+    }
+    // lane l holds column l % 32, rows 8 (e / 4) + 4 (l / 32) + e % 4 of each block
+#pragma unroll
+    for (int nb = 0; nb < 2; ++nb)
+#pragma unroll
+        for (int e = 0; e < 16; ++e) {
+            const int r = r0 + 8 * (e >> 2) + 4 * kg + (e & 3);
+            const int col = nb * 32 + i;
+            if (r < n && col < t.n_roots) {
+                int64_t dot = 0;
+#pragma unroll
+                for (int w = 0; w < 5; ++w) dot += (int64_t)acc[w][nb][e] << (8 * w);
+                sc[(int64_t)r * B * Fmax + col] = ref_score(dot, qn[r], t.sqnorm[t.roots[col]]);
+            }
+        }
+}
+
 struct DescOut {
     int32_t* path;   // [n][L]
     float* pscore;   // [n][L]
@@ -580,24 +696,32 @@ extern "C" int hvk_descend(hvk_ctx* ctx, const hvk_tree* tree, const int16_t* d_
     constexpr int kSelThreads = 128, kSelMaxLds = 64;
     const bool sel_lds = maxN <= kSelMaxLds;
     const size_t sel_bytes = sel_lds ? 2 * sizeof(float) * (size_t)maxN * kSelThreads : 0;
+    // level 0 on the matrix cores when its operands fit the digit split
+    const bool dense = tree->n_roots > 0 && tree->n_roots <= kDenseN && tree->max_abs_proto > 0 &&
+                       tree->max_abs_proto < (int64_t(1) << 30) && q_maxabs <= 32639;
     timing_mark(ctx, 1, false, st);
     hipLaunchKernelGGL(desc_init, dim3(rb), dim3(kDescThreads), 0, st, bm[0], n_q, B);
     HVK_HIP(hipGetLastError());
     int cur = 0;
     for (int lvl = 0; lvl < L; ++lvl) {
-        hipLaunchKernelGGL(desc_fill, dim3((n_keys + kDescThreads - 1) / kDescThreads),
-                           dim3(kDescThreads), 0, st, hist, (int64_t)n_keys, 0);
-        hipLaunchKernelGGL(desc_hist, dim3(rb), dim3(kDescThreads), 0, st, bm[cur], n_q, B, n_cat,
-                           hist);
-        hipLaunchKernelGGL(desc_scan, dim3(1), dim3(1024), 0, st, hist, n_keys, cursor, total);
-        hipLaunchKernelGGL(desc_scatter, dim3(rb), dim3(kDescThreads), 0, st, bm[cur], n_q, B,
-                           n_cat, cursor, pairs, pkey);
-        if (E == 8)
-            hipLaunchKernelGGL(desc_score<8>, dim3(pb), dim3(kDescThreads), 0, st, t, d_q_counts,
-                               q_ld, d_q_sqnorm, B, K, pairs, pkey, total, Fmax, sc);
-        else
-            hipLaunchKernelGGL(desc_score<2>, dim3(pb), dim3(kDescThreads), 0, st, t, d_q_counts,
-                               q_ld, d_q_sqnorm, B, K, pairs, pkey, total, Fmax, sc);
+        if (lvl == 0 && dense) {  // every read has one entry, the root
+            hipLaunchKernelGGL(desc_dense, dim3((n_q + 127) / 128), dim3(256), 0, st, t, d_q_counts,
+                               q_ld, d_q_sqnorm, n_q, B, K, Fmax, sc);
+        } else {
+            hipLaunchKernelGGL(desc_fill, dim3((n_keys + kDescThreads - 1) / kDescThreads),
+                               dim3(kDescThreads), 0, st, hist, (int64_t)n_keys, 0);
+            hipLaunchKernelGGL(desc_hist, dim3(rb), dim3(kDescThreads), 0, st, bm[cur], n_q, B,
+                               n_cat, hist);
+            hipLaunchKernelGGL(desc_scan, dim3(1), dim3(1024), 0, st, hist, n_keys, cursor, total);
+            hipLaunchKernelGGL(desc_scatter, dim3(rb), dim3(kDescThreads), 0, st, bm[cur], n_q, B,
+                               n_cat, cursor, pairs, pkey);
+            if (E == 8)
+                hipLaunchKernelGGL(desc_score<8>, dim3(pb), dim3(kDescThreads), 0, st, t,
+                                   d_q_counts, q_ld, d_q_sqnorm, B, K, pairs, pkey, total, Fmax, sc);
+            else
+                hipLaunchKernelGGL(desc_score<2>, dim3(pb), dim3(kDescThreads), 0, st, t,
+                                   d_q_counts, q_ld, d_q_sqnorm, B, K, pairs, pkey, total, Fmax, sc);
+        }
         if (sel_lds)
             hipLaunchKernelGGL(desc_select, dim3((n_q + kSelThreads - 1) / kSelThreads),
                                dim3(kSelThreads), sel_bytes, st, t, bm[cur], bm[cur ^ 1], n_q, B, L,

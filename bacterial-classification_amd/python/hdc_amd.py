@@ -68,7 +68,7 @@ class Tree(ctypes.Structure):
                 ("d_hi", ctypes.c_void_p), ("d_sqnorm", ctypes.c_void_p),
                 ("max_abs_limb", ctypes.c_int32), ("child_off", ctypes.c_void_p),
                 ("child_idx", ctypes.c_void_p), ("roots", ctypes.c_void_p),
-                ("n_roots", ctypes.c_int)]
+                ("n_roots", ctypes.c_int), ("max_abs_proto", ctypes.c_int64)]
 
 
 class GenSpec(ctypes.Structure):
@@ -527,13 +527,17 @@ class TreeProto:
     def __init__(self, ctx, tax: Taxonomy, proto, lo, hi, sqnorm, maxabs):
         self.tax, self.proto, self.lo, self.hi, self.sqnorm = tax, proto, lo, hi, sqnorm
         self.max_abs_limb = int(maxabs.max().item()) if maxabs.numel() else 0
+        roots = _torch().from_numpy(tax.roots.astype(np.int64)).to(proto.device)
+        self.max_abs_proto = (int(proto[roots].abs().max().item()) if len(tax.roots) else 0)
         self.ld = ctx.ld
+        self.use_dense = True  # tests switch the level-0 matrix-core path off
 
     def struct(self) -> Tree:
         t = self.tax
         return Tree(t.n_cat, self.ld, self.lo.data_ptr(), self.hi.data_ptr(),
                     self.sqnorm.data_ptr(), self.max_abs_limb, t.child_off.ctypes.data,
-                    t.child_idx.ctypes.data, t.roots.ctypes.data, len(t.roots))
+                    t.child_idx.ctypes.data, t.roots.ctypes.data, len(t.roots),
+                    self.max_abs_proto if self.use_dense else 0)
 
 
 def build_prototypes(ctx: Context, tax: Taxonomy, leaf_counts, leaf_cat,

@@ -249,6 +249,9 @@ typedef struct {
     const int32_t* child_idx;    /* host, [child_off[n_cat]] */
     const int32_t* roots;        /* host, [n_roots] */
     int n_roots;
+    int64_t max_abs_proto;       /* max |p| over the roots' rows, 0 = unknown; when below
+                                    2^30 and n_roots <= 64 the level-0 scores run as an
+                                    int8-limb GEMM on the matrix cores */
 } hvk_tree;
 
 /* Outputs per read (device, caller-allocated): path[levels] category ids and

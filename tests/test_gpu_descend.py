@@ -213,3 +213,22 @@ def test_descend_rejects_bad_trees():
     rc = H.lib().hvk_descend(ctx.h, ctypes.byref(t), 1, 1, 4, ctx.ld, 100, 3, 0.0, 6, 1, 1, 1, 1,
                              None)
     assert rc == H.HVK_ERR_RANGE
+
+
+def test_dense_level0_equals_vector_path(synth):
+    """Level 0 on the matrix cores (int8 digit GEMM) and on the vector path give
+    identical descents."""
+    import hdc_amd as H
+    import torch
+    ctx, tax, tree, q, proto, qc, truth = synth
+    outs = []
+    for dense in (True, False):
+        tree.use_dense = dense
+        print(f"descend synthetic dense_level0={dense}", flush=True)
+        r = H.descend(ctx, tree, q, beam=3, threshold=-1.0, levels=6)
+        torch.cuda.synchronize()
+        outs.append((r.path.cpu().numpy(), r.score.cpu().numpy().view(np.int32),
+                     r.length.cpu().numpy(), r.exit_level.cpu().numpy()))
+    tree.use_dense = True
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
