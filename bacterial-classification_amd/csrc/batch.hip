@@ -397,7 +397,7 @@ extern "C" int hvk_batch_autotune(hvk_batch* b, const uint32_t* d_packed,
     struct Cand { int form, splits, chunk; };
     std::vector<Cand> cands;
     for (int chunk : {m / 4, m / 2, m})
-        for (int s : {2, 4, 8})
+        for (int s : {4, 8, 16})
             if (chunk >= kMinChunk) cands.push_back({HVK_BATCH_FORM_BOUNDED, s, chunk});
     cands.push_back({HVK_BATCH_FORM_TWO_LIMB, 4, std::max(kMinChunk, m / 2)});
     hipStream_t st = (hipStream_t)stream;

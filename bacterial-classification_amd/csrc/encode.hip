@@ -95,6 +95,35 @@ __device__ __forceinline__ int wave_max(int v) {
     return v;
 }
 
+// Codes of chunk positions j .. j+7 (j a multiple of 8): two 16-byte reads at
+// a wave-uniform address.  Phase 1 fills the positions past the chunk up to
+// the next multiple of 8 with kInvalidCode.
+__device__ __forceinline__ void load_codes8(const uint32_t* codes, int j, uint32_t (&c)[8]) {
+    const uint4 a = *reinterpret_cast<const uint4*>(codes + j);
+    const uint4 b = *reinterpret_cast<const uint4*>(codes + j + 4);
+    c[0] = a.x; c[1] = a.y; c[2] = a.z; c[3] = a.w;
+    c[4] = b.x; c[5] = b.y; c[6] = b.z; c[7] = b.w;
+}
+
+// Phase 1: canonical code of chunk position j (kInvalidCode when a base of
+// the k-mer is not A/C/G/T or j >= cn).
+__device__ __forceinline__ uint32_t kmer_code(const uint32_t* __restrict__ packed,
+                                              const uint32_t* __restrict__ invalid, int64_t p,
+                                              int j, int cn, int k, uint32_t mask2k,
+                                              uint32_t maskk) {
+    if (j >= cn) return kInvalidCode;
+    const uint64_t w2 = (uint64_t)packed[p >> 4] | ((uint64_t)packed[(p >> 4) + 1] << 32);
+    const uint32_t v = (uint32_t)(w2 >> (2 * (p & 15))) & mask2k;
+    const uint64_t iv = (uint64_t)invalid[p >> 5] | ((uint64_t)invalid[(p >> 5) + 1] << 32);
+    const uint32_t bad = (uint32_t)(iv >> (p & 31)) & maskk;
+    if (bad) return kInvalidCode;
+    // v: digit i = base i; forward code f has base 0 most significant;
+    // revcomp code = mask - v.  Stored: rev(canonical), digit i = c_i.
+    const uint32_t f = rev_digits(v, k);
+    const uint32_t r = mask2k - v;
+    return (r < f) ? (mask2k - f) : v;
+}
+
 // Phase 3, called by every thread of the workgroup: the threads with
 // `active` set own word w of the row and hold its bit-sliced counts; they
 // write counts / limbs / clamped rows, and the workgroup reduces the exact
@@ -226,43 +255,31 @@ encode_kernel(const uint32_t* __restrict__ gtable, int k_rt, int D, int W32,
         for (int64_t c0 = 0; c0 < npos; c0 += kChunk) {
             const int cn = (int)((npos - c0) < kChunk ? (npos - c0) : kChunk);
             __syncthreads();
-            for (int j = tid; j < cn; j += nthr) {
-                const int64_t p = off + c0 + j;
-                const uint64_t w2 = (uint64_t)packed[p >> 4] | ((uint64_t)packed[(p >> 4) + 1] << 32);
-                const uint32_t v = (uint32_t)(w2 >> (2 * (p & 15))) & mask2k;
-                const uint64_t iv = (uint64_t)invalid[p >> 5] | ((uint64_t)invalid[(p >> 5) + 1] << 32);
-                const uint32_t bad = (uint32_t)(iv >> (p & 31)) & maskk;
-                uint32_t code = kInvalidCode;
-                if (!bad) {
-                    // v: digit i = base i; forward code f has base 0 most significant;
-                    // revcomp code = mask - v.  Store rev(canonical): digit i = c_i.
-                    const uint32_t f = rev_digits(v, k);
-                    const uint32_t r = mask2k - v;
-                    code = (r < f) ? (mask2k - f) : v;
-                }
-                codes[j] = code;
-            }
+            const int cn8 = (cn + 7) & ~7;
+            for (int j = tid; j < cn8; j += nthr)
+                codes[j] = kmer_code(packed, invalid, off + c0 + j, j, cn, k, mask2k, maskk);
             __syncthreads();
             if (w < W32) {
                 for (int j = 0; j < cn; j += 8) {
-                    uint32_t x[8];
+                    // branch-free: an invalid code (all digits 3) reads valid
+                    // table rows and its word is dropped afterwards
+                    uint32_t c[8], x[8];
+                    load_codes8(codes, j, c);
 #pragma unroll
                     for (int u = 0; u < 8; ++u) {
-                        uint32_t code = (j + u < cn) ? codes[j + u] : kInvalidCode;
-                        code = __builtin_amdgcn_readfirstlane(code);
+                        const uint32_t code = __builtin_amdgcn_readfirstlane(c[u]);
                         uint32_t y = 0;
-                        if (code != kInvalidCode) {
-                            ++nvalid;
-                            if (K > 0) {
+                        if (K > 0) {
 #pragma unroll
-                                for (int i = 0; i < (K > 0 ? K : 1); ++i)
-                                    y ^= tab[((code >> (2 * i)) & 3u) * kW + i * W32 + w];
-                            } else {
-                                for (int i = 0; i < k; ++i)
-                                    y ^= tab[((code >> (2 * i)) & 3u) * kW + i * W32 + w];
-                            }
+                            for (int i = 0; i < (K > 0 ? K : 1); ++i)
+                                y ^= tab[((code >> (2 * i)) & 3u) * kW + i * W32 + w];
+                        } else {
+                            for (int i = 0; i < k; ++i)
+                                y ^= tab[((code >> (2 * i)) & 3u) * kW + i * W32 + w];
                         }
-                        x[u] = y;
+                        const bool valid = code != kInvalidCode;
+                        nvalid += valid ? 1 : 0;
+                        x[u] = valid ? y : 0u;
                     }
                     add8<P>(planes, x);
                 }
@@ -332,43 +349,33 @@ encode_pair_kernel(const uint32_t* __restrict__ gtable, int k_rt, int D, int W32
         for (int64_t c0 = 0; c0 < npos; c0 += kChunk) {
             const int cn = (int)((npos - c0) < kChunk ? (npos - c0) : kChunk);
             __syncthreads();
-            for (int j = tid; j < cn; j += nthr) {
-                const int64_t p = off + c0 + j;
-                const uint64_t w2 = (uint64_t)packed[p >> 4] | ((uint64_t)packed[(p >> 4) + 1] << 32);
-                const uint32_t v = (uint32_t)(w2 >> (2 * (p & 15))) & mask2k;
-                const uint64_t iv = (uint64_t)invalid[p >> 5] | ((uint64_t)invalid[(p >> 5) + 1] << 32);
-                const uint32_t bad = (uint32_t)(iv >> (p & 31)) & maskk;
-                uint32_t code = kInvalidCode;
-                if (!bad) {
-                    const uint32_t f = rev_digits(v, k);
-                    const uint32_t r = mask2k - v;
-                    code = (r < f) ? (mask2k - f) : v;
-                }
-                codes[j] = code;
-            }
+            const int cn8 = (cn + 7) & ~7;
+            for (int j = tid; j < cn8; j += nthr)
+                codes[j] = kmer_code(packed, invalid, off + c0 + j, j, cn, k, mask2k, maskk);
             __syncthreads();
             if (w < W32) {
                 for (int j = 8 * half; j < cn; j += 16) {  // alternate 8-blocks per half
-                    uint32_t x[8];
+                    // branch-free: an invalid code (all digits 3) reads valid
+                    // table rows and its word is dropped afterwards
+                    uint32_t c[8], x[8];
+                    load_codes8(codes, j, c);
 #pragma unroll
                     for (int u = 0; u < 8; ++u) {
-                        uint32_t code = (j + u < cn) ? codes[j + u] : kInvalidCode;
-                        code = __builtin_amdgcn_readfirstlane(code);
+                        const uint32_t code = __builtin_amdgcn_readfirstlane(c[u]);
                         uint32_t y = 0;
-                        if (code != kInvalidCode) {
-                            ++nvalid;
-                            if (K > 0) {
+                        if (K > 0) {
 #pragma unroll
-                                for (int jj = 0; jj < (K > 0 ? K / 2 : 1); ++jj)
-                                    y ^= tab2[((code >> (4 * jj)) & 15u) * KHW + jj * W32 + w];
-                                if (K & 1) y ^= tab1[((code >> (2 * (k - 1))) & 3u) * W32 + w];
-                            } else {
-                                for (int jj = 0; jj < KH; ++jj)
-                                    y ^= tab2[((code >> (4 * jj)) & 15u) * KHW + jj * W32 + w];
-                                if (k & 1) y ^= tab1[((code >> (2 * (k - 1))) & 3u) * W32 + w];
-                            }
+                            for (int jj = 0; jj < (K > 0 ? K / 2 : 1); ++jj)
+                                y ^= tab2[((code >> (4 * jj)) & 15u) * KHW + jj * W32 + w];
+                            if (K & 1) y ^= tab1[((code >> (2 * (k - 1))) & 3u) * W32 + w];
+                        } else {
+                            for (int jj = 0; jj < KH; ++jj)
+                                y ^= tab2[((code >> (4 * jj)) & 15u) * KHW + jj * W32 + w];
+                            if (k & 1) y ^= tab1[((code >> (2 * (k - 1))) & 3u) * W32 + w];
                         }
-                        x[u] = y;
+                        const bool valid = code != kInvalidCode;
+                        nvalid += valid ? 1 : 0;
+                        x[u] = valid ? y : 0u;
                     }
                     add8<P>(planes, x);
                 }

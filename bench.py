@@ -62,7 +62,7 @@ def parse():
                          "descent: top-3 beam down a 7-rank taxonomy of 60,000 leaves, 1M reads; "
                          "batch: the batching form on 4M reads of 100-1,600 nt, 100k refs, top-5")
     ap.add_argument("--batch-reads", type=int, default=4_000_000)
-    ap.add_argument("--batch-sample", type=int, default=262144, help="autotuner sample")
+    ap.add_argument("--batch-sample", type=int, default=1 << 20, help="autotuner sample")
     ap.add_argument("--descent-reads", type=int, default=1_000_000)
     ap.add_argument("--descent-beam", type=int, default=3)
     ap.add_argument("--precision-reads", type=int, default=2_000_000)
