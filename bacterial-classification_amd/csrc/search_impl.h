@@ -676,15 +676,20 @@ mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
         };
         // First tile of the slice: the lists are empty, and without a
         // threshold nearly every pair would be offered.  Raise each query's
-        // threshold to a floor first: the lanes of a half-wave are split into
-        // KP groups (32 / KP lanes, WY references each); the minimum over
-        // the groups of each group's best score is reached by KP distinct
-        // references of the tile, so it does not exceed the query's KP-th
-        // best key, and keys below it could not enter the top KP.
+        // threshold to a floor first: the tile's NT references of a query
+        // (the kMfmaGN waves of its row group, 32 lanes x WY each) are split
+        // into KP disjoint groups of GL lanes x WY references; the minimum
+        // over the groups of each group's best score is reached by KP
+        // distinct references of the tile, so it does not exceed the query's
+        // KP-th best key, and keys below it could not enter the top KP.
+        // (Groups of 16 references at KP = 16 leave ~48 of 256 references
+        // per query above the floor on random scores.)
         if (n0 == n_begin) {
             // (the reference stages hold the floors; the offers below reuse them)
             uint32_t* floor_o = reinterpret_cast<uint32_t*>(&sm.r[0][0][0]);
-            for (int i = tid; i < MT; i += NTH) floor_o[i] = 0;
+            constexpr int GL = 32 * kMfmaGN / KP;  // lanes per group
+            static_assert(KP % kMfmaGN == 0 && GL >= 1 && GL <= 32, "floor groups");
+            for (int i = tid; i < MT; i += NTH) floor_o[i] = 0xFFFFFFFFu;
             __syncthreads();
             static_for<WX>([&](auto X) {
                 constexpr int x = decltype(X)::value;
@@ -701,9 +706,9 @@ mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
                         const float sc = pair_score(X, Y, E, iqh, bqh);
                         if (n < nr) v = fmaxf(v, sc);
                     });
-                    for (int o = 1; o < 32 / KP; o <<= 1) v = fmaxf(v, __shfl_xor(v, o));
-                    for (int o = 1; o < 32; o <<= 1) v = fminf(v, __shfl_xor(v, o));
-                    if ((lane_e & 31) == 0) atomicMax(&floor_o[row], ord_f32(v));
+                    for (int o = 1; o < GL; o <<= 1) v = fmaxf(v, __shfl_xor(v, o));
+                    for (int o = GL; o < 32; o <<= 1) v = fminf(v, __shfl_xor(v, o));
+                    if ((lane_e & 31) == 0) atomicMin(&floor_o[row], ord_f32(v));
                 });
             });
             __syncthreads();
