@@ -294,12 +294,13 @@ encode_kernel(const uint32_t* __restrict__ gtable, int k_rt, int D, int W32,
 // rot(base[c_2j], 2j) ^ rot(base[c_2j+1], 2j+1) halves the LDS lookups and
 // address adds per k-mer (6 instead of 12 at k = 12).  The table is 4x the
 // single one (123 KB at D = 10240, k = 12), so one workgroup fills a CU; to
-// keep ten waves per CU, two halves of the workgroup share each sequence:
-// they take alternate blocks of 8 k-mers into their own bit-sliced counters,
-// which are added (bit-sliced) before phase 3.
+// keep more waves per CU, NP parts of the workgroup (as many as 1024 threads
+// hold, at most 4) share each sequence: they take alternate blocks of 8
+// k-mers into their own bit-sliced counters, which are added (bit-sliced)
+// into part 0's before phase 3.
 template <int K, int P>
 __global__ void __launch_bounds__(1024)
-encode_pair_kernel(const uint32_t* __restrict__ gtable, int k_rt, int D, int W32, int TW,
+encode_pair_kernel(const uint32_t* __restrict__ gtable, int k_rt, int D, int W32, int TW, int NP,
                    const uint32_t* __restrict__ packed, const uint32_t* __restrict__ invalid,
                    const int64_t* __restrict__ offsets, const int64_t* __restrict__ starts,
                    const int32_t* __restrict__ lens, int window_len, int n_seqs,
@@ -318,8 +319,8 @@ encode_pair_kernel(const uint32_t* __restrict__ gtable, int k_rt, int D, int W32
 
     const int tid = threadIdx.x;
     const int nthr = blockDim.x;
-    const int half = tid >= TW ? 1 : 0;
-    const int w = tid - half * TW;
+    const int part = tid / TW;
+    const int w = tid - part * TW;
     const int kW = k * W32;
     const int KHW = KH * W32;
     for (int i = tid; i < 16 * KHW; i += nthr) {
@@ -354,7 +355,7 @@ encode_pair_kernel(const uint32_t* __restrict__ gtable, int k_rt, int D, int W32
                 codes[j] = kmer_code(packed, invalid, off + c0 + j, j, cn, k, mask2k, maskk);
             __syncthreads();
             if (w < W32) {
-                for (int j = 8 * half; j < cn; j += 16) {  // alternate 8-blocks per half
+                for (int j = 8 * part; j < cn; j += 8 * NP) {  // alternate 8-blocks per part
                     // branch-free: an invalid code (all digits 3) reads valid
                     // table rows and its word is dropped afterwards
                     uint32_t c[8], x[8];
@@ -381,26 +382,28 @@ encode_pair_kernel(const uint32_t* __restrict__ gtable, int k_rt, int D, int W32
                 }
             }
         }
-        // add half 1's counters into half 0's (bit-sliced ripple add)
-        __syncthreads();
-        if (half == 1 && w < W32) {
+        // add the other parts' counters into part 0's (bit-sliced ripple add)
+        for (int q = 1; q < NP; ++q) {
+            __syncthreads();
+            if (part == q && w < W32) {
 #pragma unroll
-            for (int p = 0; p < P; ++p) xplanes[p * W32 + w] = planes[p];
-            if (w == 0) s_nvalid = nvalid;
-        }
-        __syncthreads();
-        if (half == 0 && w < W32) {
-            uint32_t cy = 0;
-#pragma unroll
-            for (int p = 0; p < P; ++p) {
-                const uint32_t a = planes[p], b = xplanes[p * W32 + w];
-                const uint32_t t = a ^ b;
-                planes[p] = t ^ cy;
-                cy = (a & b) | (t & cy);
+                for (int p = 0; p < P; ++p) xplanes[p * W32 + w] = planes[p];
+                if (w == 0) s_nvalid = nvalid;
             }
-            nvalid += s_nvalid;
+            __syncthreads();
+            if (part == 0 && w < W32) {
+                uint32_t cy = 0;
+#pragma unroll
+                for (int p = 0; p < P; ++p) {
+                    const uint32_t a = planes[p], b = xplanes[p * W32 + w];
+                    const uint32_t t = a ^ b;
+                    planes[p] = t ^ cy;
+                    cy = (a & b) | (t & cy);
+                }
+                nvalid += s_nvalid;
+            }
         }
-        finish_row<P>(planes, half == 0 && w < W32, w, nvalid, D, W32, s, out, red_sq, red_mx,
+        finish_row<P>(planes, part == 0 && w < W32, w, nvalid, D, W32, s, out, red_sq, red_mx,
                       red_res);
     }
 }
@@ -412,18 +415,19 @@ int launch_kp(hvk_ctx* ctx, const uint32_t* packed, const uint32_t* invalid,
     const int W32 = ctx->w32;
     const int threads = (W32 + 63) / 64 * 64;
     const int k = ctx->k;
-    // pair table when its 16 x (k/2) rows, two halves of threads and the half
-    // counters fit one workgroup
+    // pair table when its 16 x (k/2) rows, the codes and one part's counters
+    // fit one workgroup; as many parts (<= 4) as 1024 threads hold
     const size_t lds_pair = sizeof(uint32_t) * ((size_t)16 * (k / 2) * W32 +
                                                 ((k & 1) ? 4 * (size_t)W32 : 0) + kChunk +
                                                 (size_t)P * W32);
-    const bool pair = k >= 2 && 2 * threads <= 1024 && lds_pair + 512 <= (size_t)ctx->max_lds;
+    const int np = std::min(4, 1024 / threads);
+    const bool pair = k >= 2 && np >= 1 && lds_pair + 512 <= (size_t)ctx->max_lds;
     int grid = n < 4 * ctx->n_cu ? n : 4 * ctx->n_cu;
     timing_mark(ctx, 0, false, st);
     if (pair) {
-        hipLaunchKernelGGL((encode_pair_kernel<K, P>), dim3(grid), dim3(2 * threads), lds_pair, st,
-                           ctx->d_table, k, ctx->dim, W32, threads, packed, invalid, offsets,
-                           starts, lens, window_len, n, out);
+        hipLaunchKernelGGL((encode_pair_kernel<K, P>), dim3(grid), dim3(np * threads), lds_pair,
+                           st, ctx->d_table, k, ctx->dim, W32, threads, np, packed, invalid,
+                           offsets, starts, lens, window_len, n, out);
     } else {
         const size_t lds = sizeof(uint32_t) * ((size_t)4 * k * W32 + kChunk);
         if (lds + 256 > (size_t)ctx->max_lds) return HVK_ERR_ARG;
