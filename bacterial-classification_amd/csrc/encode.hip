@@ -124,6 +124,42 @@ __device__ __forceinline__ uint32_t kmer_code(const uint32_t* __restrict__ packe
     return (r < f) ? (mask2k - f) : v;
 }
 
+typedef short i16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+// Even bits of x to bits 0..15, odd bits to 16..31 (inverse outer perfect
+// shuffle, four delta swaps).
+__device__ __forceinline__ uint32_t unzip_bits(uint32_t x) {
+    uint32_t t;
+    t = (x ^ (x >> 1)) & 0x22222222u; x ^= t ^ (t << 1);
+    t = (x ^ (x >> 2)) & 0x0C0C0C0Cu; x ^= t ^ (t << 2);
+    t = (x ^ (x >> 4)) & 0x00F000F0u; x ^= t ^ (t << 4);
+    t = (x ^ (x >> 8)) & 0x0000FF00u; x ^= t ^ (t << 8);
+    return x;
+}
+
+// Two 16 x 16 bit-matrix transposes at once (one per 16-bit half): on
+// return bit p of half h of w[i] is bit i of half h of the input w[p].
+__device__ __forceinline__ void transpose16x2(uint32_t (&w)[16]) {
+#pragma unroll
+    for (int sh = 8; sh >= 1; sh >>= 1) {
+        const uint32_t m = sh == 8 ? 0x00FF00FFu : sh == 4 ? 0x0F0F0F0Fu
+                         : sh == 2 ? 0x33333333u : 0x55555555u;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+            if (j & sh) continue;
+            const uint32_t t = ((w[j] >> sh) ^ w[j + sh]) & m;
+            w[j + sh] ^= t;
+            w[j] ^= t << sh;
+        }
+    }
+}
+
+__device__ __forceinline__ u16x2 as_u16x2(uint32_t v) { return __builtin_bit_cast(u16x2, v); }
+__device__ __forceinline__ i16x2 as_i16x2(uint32_t v) { return __builtin_bit_cast(i16x2, v); }
+__device__ __forceinline__ uint32_t as_u32(u16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ uint32_t as_u32(i16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+
 // Phase 3, called by every thread of the workgroup: the threads with
 // `active` set own word w of the row and hold its bit-sliced counts; they
 // write counts / limbs / clamped rows, and the workgroup reduces the exact
@@ -142,6 +178,52 @@ __device__ __forceinline__ void finish_row(const uint32_t (&planes)[P], bool act
     if (active) {
         uint32_t packed16[16];
         uint32_t plo[8], phi[8], pcl[8];
+        if constexpr (P <= 16) {
+            // Word i of the transposed planes holds the popcounts of dims 2i
+            // (low half) and 2i+1 (high half); the rest is packed 16-bit
+            // arithmetic, modulo 2^16 as the int16 counts are.
+            uint32_t t[16];
+#pragma unroll
+            for (int p = 0; p < 16; ++p) t[p] = p < P ? unzip_bits(planes[p]) : 0u;
+            transpose16x2(t);
+            const int lim = D - 32 * w;  // dims >= lim of this word are padding
+            const u16x2 nv2 = {(unsigned short)nvalid, (unsigned short)nvalid};
+            const u16x2 c127 = {127, 127}, c128 = {128, 128}, ones = {1, 1};
+            const i16x2 p127 = {127, 127}, m127 = {-127, -127};
+            u16x2 mx2 = {0, 0};
+            uint32_t res_u = 0;
+            int sq32 = 0;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const u16x2 pc = as_u16x2(t[i]);
+                u16x2 c = nv2 - (pc + pc);
+                if (lim < 32) {
+                    if (2 * i >= lim) c.x = 0;
+                    if (2 * i + 1 >= lim) c.y = 0;
+                }
+                const i16x2 cs = as_i16x2(as_u32(c));
+                if constexpr (P <= 11) sq32 = __builtin_amdgcn_sdot2(cs, cs, sq32, false);
+                else sq += (int64_t)(uint32_t)__builtin_amdgcn_sdot2(cs, cs, 0, false);
+                const u16x2 zero = {0, 0};
+                const u16x2 a = as_u16x2(as_u32(__builtin_elementwise_max(cs, as_i16x2(as_u32(zero - c)))));
+                mx2 = __builtin_elementwise_max(mx2, a);
+                res_u = __builtin_amdgcn_udot2(__builtin_elementwise_sub_sat(a, c127), ones, res_u,
+                                               false);
+                packed16[i] = as_u32(c);
+                t[i] = as_u32(c + c128);  // high bytes: the hi limb
+                const i16x2 cl = __builtin_elementwise_max(__builtin_elementwise_min(cs, p127), m127);
+                if (i & 1) {
+                    plo[i >> 1] = __builtin_amdgcn_perm(packed16[i], packed16[i - 1], 0x06040200u);
+                    phi[i >> 1] = __builtin_amdgcn_perm(t[i], t[i - 1], 0x07050301u);
+                    pcl[i >> 1] = __builtin_amdgcn_perm(as_u32(cl), pcl[i >> 1], 0x06040200u);
+                } else {
+                    pcl[i >> 1] = as_u32(cl);
+                }
+            }
+            if constexpr (P <= 11) sq = sq32;
+            mx = max((int)mx2.x, (int)mx2.y);
+            res = (int)res_u;
+        } else {
 #pragma unroll
         for (int q = 0; q < 8; ++q) { plo[q] = 0; phi[q] = 0; pcl[q] = 0; }
 #pragma unroll
@@ -163,6 +245,7 @@ __device__ __forceinline__ void finish_row(const uint32_t (&planes)[P], bool act
             plo[b >> 2] |= ((uint32_t)(uint8_t)l8) << (8 * (b & 3));
             phi[b >> 2] |= ((uint32_t)(uint8_t)h8) << (8 * (b & 3));
             pcl[b >> 2] |= ((uint32_t)(uint8_t)c8) << (8 * (b & 3));
+        }
         }
         const int64_t row = s * ld + 32 * (int64_t)w;
         if (out.counts) {
