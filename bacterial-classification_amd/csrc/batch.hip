@@ -405,18 +405,22 @@ extern "C" int hvk_batch_autotune(hvk_batch* b, const uint32_t* d_packed,
     Cand pick = {b->form, b->splits, b->chunk};
     for (const Cand& c : cands) {
         if (int e = hvk_batch_config(b, c.form, c.splits, c.chunk)) return e;
-        // direct run + capture, then the timed replay
+        // direct run + capture, then the faster of two timed replays
         if (int e = classify(b, d_packed, d_invalid, h_offsets, m, *db, k, d_idx, d_score, nullptr,
                              nullptr, st))
             return e;
-        HVK_HIP(hipEventRecord(b->t0, st));
-        if (int e = classify(b, d_packed, d_invalid, h_offsets, m, *db, k, d_idx, d_score, nullptr,
-                             nullptr, st))
-            return e;
-        HVK_HIP(hipEventRecord(b->t1, st));
-        HVK_HIP(hipEventSynchronize(b->t1));
-        float ms = 0.0f;
-        HVK_HIP(hipEventElapsedTime(&ms, b->t0, b->t1));
+        float ms = -1.0f;
+        for (int rep = 0; rep < 2; ++rep) {
+            HVK_HIP(hipEventRecord(b->t0, st));
+            if (int e = classify(b, d_packed, d_invalid, h_offsets, m, *db, k, d_idx, d_score,
+                                 nullptr, nullptr, st))
+                return e;
+            HVK_HIP(hipEventRecord(b->t1, st));
+            HVK_HIP(hipEventSynchronize(b->t1));
+            float t = 0.0f;
+            HVK_HIP(hipEventElapsedTime(&t, b->t0, b->t1));
+            if (ms < 0.0f || t < ms) ms = t;
+        }
         if (best < 0.0f || ms < best) {
             best = ms;
             pick = c;
