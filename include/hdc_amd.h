@@ -357,9 +357,12 @@ int hvk_batch_classify(hvk_batch* b, const uint32_t* d_packed, const uint32_t* d
                        int32_t* d_idx, float* d_score, double* d_cos64, int32_t* d_n_rescored,
                        hvk_stream stream);
 /* Times each candidate setting on the first min(sample, n) reads of the
- * batch (one direct run, then the timed graph replay, into the caller's
- * output buffers) and applies the fastest; *ms_per_mread (nullable) gets its
- * time per million reads.  Synchronises `stream`. */
+ * batch (one direct run, then two timed graph replays, the faster counted,
+ * into the caller's output buffers) and applies the fastest; *ms_per_mread
+ * (nullable) gets its time per million reads.  Candidates: the bounded form
+ * with 4, 8 or 16 reference slices at chunks of 1/4, 1/2 and all of the
+ * sample (at least 256 reads), and the two-limb form.  Synchronises
+ * `stream`. */
 int hvk_batch_autotune(hvk_batch* b, const uint32_t* d_packed, const uint32_t* d_invalid,
                        const int64_t* h_offsets, int n, const hvk_bundles* db, int k,
                        int32_t* d_idx, float* d_score, int sample, float* ms_per_mread,
