@@ -678,17 +678,19 @@ mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
         // threshold nearly every pair would be offered.  Raise each query's
         // threshold to a floor first: the tile's NT references of a query
         // (the kMfmaGN waves of its row group, 32 lanes x WY each) are split
-        // into KP disjoint groups of GL lanes x WY references; the minimum
-        // over the groups of each group's best score is reached by KP
-        // distinct references of the tile, so it does not exceed the query's
-        // KP-th best key, and keys below it could not enter the top KP.
-        // (Groups of 16 references at KP = 16 leave ~48 of 256 references
-        // per query above the floor on random scores.)
+        // into KP / 2 disjoint groups of GL lanes x WY references; each
+        // group's two best scores belong to two distinct references, so the
+        // minimum over the groups of each group's second-best score is
+        // reached by KP distinct references of the tile: it does not exceed
+        // the query's KP-th best key, and keys below it could not enter the
+        // top KP.  (Groups of 32 references at KP = 16 leave ~33 of 256
+        // references per query above the floor on random scores; the maxima
+        // of 16 groups of 16 leave ~48.)
         if (n0 == n_begin) {
             // (the reference stages hold the floors; the offers below reuse them)
             uint32_t* floor_o = reinterpret_cast<uint32_t*>(&sm.r[0][0][0]);
-            constexpr int GL = 32 * kMfmaGN / KP;  // lanes per group
-            static_assert(KP % kMfmaGN == 0 && GL >= 1 && GL <= 32, "floor groups");
+            constexpr int GL = 2 * 32 * kMfmaGN / KP;  // lanes per group
+            static_assert(KP % (2 * kMfmaGN) == 0 && GL >= 1 && GL <= 32, "floor groups");
             for (int i = tid; i < MT; i += NTH) floor_o[i] = 0xFFFFFFFFu;
             __syncthreads();
             static_for<WX>([&](auto X) {
@@ -699,14 +701,19 @@ mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
                     const int row = mb + 8 * (e >> 2) + (e & 3);
                     const float iqh = sm.iq[row];
                     const float2 bqh = sm.bq[row];
-                    float v = -INFINITY;
+                    float a = -INFINITY, v = -INFINITY;  // best, second best
                     static_for<WY>([&](auto Y) {
                         constexpr int y = decltype(Y)::value;
                         const int n = n0 + wn * 32 * WY + y * 32 + (lane_e & 31);
-                        const float sc = pair_score(X, Y, E, iqh, bqh);
-                        if (n < nr) v = fmaxf(v, sc);
+                        const float sc = n < nr ? pair_score(X, Y, E, iqh, bqh) : -INFINITY;
+                        v = fmaxf(v, fminf(a, sc));
+                        a = fmaxf(a, sc);
                     });
-                    for (int o = 1; o < GL; o <<= 1) v = fmaxf(v, __shfl_xor(v, o));
+                    for (int o = 1; o < GL; o <<= 1) {
+                        const float a2 = __shfl_xor(a, o), v2 = __shfl_xor(v, o);
+                        v = fmaxf(fminf(a, a2), fmaxf(v, v2));
+                        a = fmaxf(a, a2);
+                    }
                     for (int o = GL; o < 32; o <<= 1) v = fminf(v, __shfl_xor(v, o));
                     if ((lane_e & 31) == 0) atomicMin(&floor_o[row], ord_f32(v));
                 });
