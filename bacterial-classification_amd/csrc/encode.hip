@@ -107,14 +107,35 @@ __device__ __forceinline__ void load_codes8(const uint32_t* codes, int j, uint32
 
 // Phase 1: canonical code of chunk position j (kInvalidCode when a base of
 // the k-mer is not A/C/G/T or j >= cn).
+// The 64-bit windows of the 2-bit packed bases and the invalid-base bits that
+// hold position p.
+__device__ __forceinline__ uint64_t base_window(const uint32_t* __restrict__ packed, int64_t p) {
+    return (uint64_t)packed[p >> 4] | ((uint64_t)packed[(p >> 4) + 1] << 32);
+}
+__device__ __forceinline__ uint64_t invalid_window(const uint32_t* __restrict__ invalid,
+                                                   int64_t p) {
+    return (uint64_t)invalid[p >> 5] | ((uint64_t)invalid[(p >> 5) + 1] << 32);
+}
+
+// Canonical code of the k-mer at position p from its windows.
+__device__ __forceinline__ uint32_t code_of_windows(uint64_t w2, uint64_t iv, int64_t p, int k,
+                                                    uint32_t mask2k, uint32_t maskk) {
+    const uint32_t v = (uint32_t)(w2 >> (2 * (p & 15))) & mask2k;
+    const uint32_t bad = (uint32_t)(iv >> (p & 31)) & maskk;
+    if (bad) return kInvalidCode;
+    const uint32_t f = rev_digits(v, k);
+    const uint32_t r = mask2k - v;
+    return (r < f) ? (mask2k - f) : v;
+}
+
 __device__ __forceinline__ uint32_t kmer_code(const uint32_t* __restrict__ packed,
                                               const uint32_t* __restrict__ invalid, int64_t p,
                                               int j, int cn, int k, uint32_t mask2k,
                                               uint32_t maskk) {
     if (j >= cn) return kInvalidCode;
-    const uint64_t w2 = (uint64_t)packed[p >> 4] | ((uint64_t)packed[(p >> 4) + 1] << 32);
+    const uint64_t w2 = base_window(packed, p);
     const uint32_t v = (uint32_t)(w2 >> (2 * (p & 15))) & mask2k;
-    const uint64_t iv = (uint64_t)invalid[p >> 5] | ((uint64_t)invalid[(p >> 5) + 1] << 32);
+    const uint64_t iv = invalid_window(invalid, p);
     const uint32_t bad = (uint32_t)(iv >> (p & 31)) & maskk;
     if (bad) return kInvalidCode;
     // v: digit i = base i; forward code f has base 0 most significant;
@@ -420,10 +441,23 @@ encode_pair_kernel(const uint32_t* __restrict__ gtable, int k_rt, int D, int W32
     const uint32_t mask2k = (k == 16) ? 0xFFFFFFFFu : ((1u << (2 * k)) - 1u);
     const uint32_t maskk = (1u << k) - 1u;
 
+    // The next sequence's bounds are loaded at the start of a sequence, and
+    // the windows of its first chunk's position tid after this sequence's
+    // chunks, so that both loads overlap this sequence's work.
+    auto bounds = [&](int q, int64_t& o, int64_t& l) {
+        o = starts ? starts[q] : offsets[q];
+        l = starts ? (lens ? (int64_t)lens[q] : (int64_t)window_len) : offsets[q + 1] - o;
+    };
+    int64_t off_n = 0, len_n = 0;
+    uint64_t pw_n = 0, iv_n = 0;  // windows of the next sequence's position tid
+    bool pre_n = false;           // ... when that position is a k-mer of it
+    if (blockIdx.x < n_seqs) bounds(blockIdx.x, off_n, len_n);
     for (int s = blockIdx.x; s < n_seqs; s += gridDim.x) {
-        const int64_t off = starts ? starts[s] : offsets[s];
-        const int64_t len = starts ? (lens ? (int64_t)lens[s] : (int64_t)window_len)
-                                   : offsets[s + 1] - off;
+        const int64_t off = off_n, len = len_n;
+        const uint64_t pw = pw_n, iv = iv_n;
+        const bool pre = pre_n;
+        const int s2 = s + gridDim.x;
+        if (s2 < n_seqs) bounds(s2, off_n, len_n);
         const int64_t npos = len - k + 1;
         uint32_t planes[P];
 #pragma unroll
@@ -435,7 +469,9 @@ encode_pair_kernel(const uint32_t* __restrict__ gtable, int k_rt, int D, int W32
             __syncthreads();
             const int cn8 = (cn + 7) & ~7;
             for (int j = tid; j < cn8; j += nthr)
-                codes[j] = kmer_code(packed, invalid, off + c0 + j, j, cn, k, mask2k, maskk);
+                codes[j] = (c0 == 0 && j == tid && pre)
+                               ? code_of_windows(pw, iv, off + j, k, mask2k, maskk)
+                               : kmer_code(packed, invalid, off + c0 + j, j, cn, k, mask2k, maskk);
             __syncthreads();
             if (w < W32) {
                 for (int j = 8 * part; j < cn; j += 8 * NP) {  // alternate 8-blocks per part
@@ -464,6 +500,12 @@ encode_pair_kernel(const uint32_t* __restrict__ gtable, int k_rt, int D, int W32
                     add8<P>(planes, x);
                 }
             }
+        }
+        pre_n = false;
+        if (s2 < n_seqs && tid < kChunk && tid < len_n - k + 1) {
+            pw_n = base_window(packed, off_n + tid);
+            iv_n = invalid_window(invalid, off_n + tid);
+            pre_n = true;
         }
         // add the other parts' counters into part 0's (bit-sliced ripple add)
         for (int q = 1; q < NP; ++q) {
