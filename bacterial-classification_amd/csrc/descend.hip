@@ -128,28 +128,83 @@ __global__ void desc_hist(Beams b, int n, int B, int n_cat, int32_t* hist) {
     }
 }
 
-__global__ void __launch_bounds__(1024)
-desc_scan(const int32_t* hist, int n_keys, int32_t* cursor, int32_t* total) {
-    __shared__ int32_t s[1024];
-    const int t = threadIdx.x;
-    const int per = (n_keys + 1023) / 1024;
-    const int a = min(t * per, n_keys), e = min(a + per, n_keys);
-    int sum = 0;
-    for (int i = a; i < e; ++i) sum += hist[i];
-    s[t] = sum;
+// Exclusive scan of the bucket counters in three launches: per-tile sums,
+// a one-workgroup scan of the tile sums (and the total), then each tile's
+// scan plus its offset.  A tile is 1024 threads x 4 consecutive counters.
+constexpr int kScanTile = 4096;
+
+// inclusive scan of v over the workgroup's 1024 threads (16 waves)
+__device__ __forceinline__ int block_scan_incl(int v, int32_t* s16) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(v, o, 64);
+        if (lane >= o) v += u;
+    }
+    if (lane == 63) s16[wv] = v;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        const int v = t >= off ? s[t - off] : 0;
+    if (threadIdx.x < 64) {
+        int x = threadIdx.x < 16 ? s16[threadIdx.x] : 0;
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) {
+            const int u = __shfl_up(x, o, 64);
+            if (lane >= o) x += u;
+        }
+        if (threadIdx.x < 16) s16[threadIdx.x] = x;  // inclusive wave totals
+    }
+    __syncthreads();
+    const int before = wv > 0 ? s16[wv - 1] : 0;
+    __syncthreads();  // s16 may be reused by the caller
+    return v + before;
+}
+
+__global__ void __launch_bounds__(1024)
+desc_scan_sums(const int32_t* __restrict__ hist, int n_keys, int32_t* __restrict__ part) {
+    __shared__ int32_t s16[16];
+    const int64_t j = (int64_t)blockIdx.x * kScanTile + 4 * threadIdx.x;
+    int v = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) v += j + q < n_keys ? hist[j + q] : 0;
+    v = block_scan_incl(v, s16);
+    if (threadIdx.x == 1023) part[blockIdx.x] = v;
+}
+
+__global__ void __launch_bounds__(1024)
+desc_scan_top(int32_t* part, int nb, int32_t* total) {
+    __shared__ int32_t s16[16];
+    __shared__ int32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int b0 = 0; b0 < nb; b0 += 1024) {
+        const int b = b0 + threadIdx.x;
+        const int v = b < nb ? part[b] : 0;
+        const int incl = block_scan_incl(v, s16);
+        const int c = carry;
+        if (b < nb) part[b] = c + incl - v;  // exclusive
         __syncthreads();
-        s[t] += v;
+        if (threadIdx.x == 1023) carry = c + incl;
         __syncthreads();
     }
-    int run = s[t] - sum;
-    for (int i = a; i < e; ++i) {
-        cursor[i] = run;
-        run += hist[i];
+    if (threadIdx.x == 0) *total = carry;
+}
+
+__global__ void __launch_bounds__(1024)
+desc_scan_apply(const int32_t* __restrict__ hist, int n_keys, const int32_t* __restrict__ part,
+                int32_t* __restrict__ cursor) {
+    __shared__ int32_t s16[16];
+    const int64_t j = (int64_t)blockIdx.x * kScanTile + 4 * threadIdx.x;
+    int h[4], v = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        h[q] = j + q < n_keys ? hist[j + q] : 0;
+        v += h[q];
     }
-    if (t == 1023) *total = s[1023];
+    int run = part[blockIdx.x] + block_scan_incl(v, s16) - v;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (j + q < n_keys) cursor[j + q] = run;
+        run += h[q];
+    }
 }
 
 __global__ void desc_scatter(Beams b, int n, int B, int n_cat, int32_t* cursor, int32_t* pairs,
@@ -664,7 +719,10 @@ extern "C" int hvk_descend(hvk_ctx* ctx, const hvk_tree* tree, const int16_t* d_
     const size_t beam_bytes = align_up(n * 4) + align_up(n * B * 4) + 2 * align_up(n * B * L * 4);
     const size_t sc_bytes = align_up((size_t)n * B * Fmax * 4);
     const size_t pair_bytes = 2 * align_up((size_t)n * B * 4);
-    const size_t key_bytes = 2 * align_up(((size_t)n_cat + 1) * kDescCopies * 4) + 256;
+    const int n_keys = (n_cat + 1) * kDescCopies;  // counter slots
+    const int n_scan = (n_keys + kScanTile - 1) / kScanTile;  // scan tiles
+    const size_t key_bytes = 2 * align_up(((size_t)n_cat + 1) * kDescCopies * 4) + 256 +
+                             align_up((size_t)n_scan * 4);
     const size_t sort_bytes = 2 * align_up((size_t)n * maxN * 4);
     if (int e = ctx->ws_desc.ensure(2 * beam_bytes + sc_bytes + pair_bytes + key_bytes + sort_bytes))
         return e;
@@ -683,13 +741,13 @@ extern "C" int hvk_descend(hvk_ctx* ctx, const hvk_tree* tree, const int16_t* d_
     int32_t* hist = (int32_t*)take(((size_t)n_cat + 1) * kDescCopies * 4);
     int32_t* cursor = (int32_t*)take(((size_t)n_cat + 1) * kDescCopies * 4);
     int32_t* total = (int32_t*)take(256);
+    int32_t* scan_part = (int32_t*)take((size_t)n_scan * 4);
     float* ws_s = (float*)take((size_t)n * maxN * 4);
     int32_t* ws_c = (int32_t*)take((size_t)n * maxN * 4);
 
     const DescOut out{d_path, d_path_score, d_path_len, d_exit_level};
     const int rb = (int)((n + kDescThreads - 1) / kDescThreads);
     const int pb = (int)((n * B + kDescP * (kDescThreads / 64) - 1) / (kDescP * (kDescThreads / 64)));
-    const int n_keys = (n_cat + 1) * kDescCopies;  // counter slots
     const int64_t K = std::min(q_ld, tld);
 
     // candidate lists of up to kSelMaxLds entries are sorted in LDS
@@ -712,7 +770,11 @@ extern "C" int hvk_descend(hvk_ctx* ctx, const hvk_tree* tree, const int16_t* d_
                                dim3(kDescThreads), 0, st, hist, (int64_t)n_keys, 0);
             hipLaunchKernelGGL(desc_hist, dim3(rb), dim3(kDescThreads), 0, st, bm[cur], n_q, B,
                                n_cat, hist);
-            hipLaunchKernelGGL(desc_scan, dim3(1), dim3(1024), 0, st, hist, n_keys, cursor, total);
+            hipLaunchKernelGGL(desc_scan_sums, dim3(n_scan), dim3(1024), 0, st, hist, n_keys,
+                               scan_part);
+            hipLaunchKernelGGL(desc_scan_top, dim3(1), dim3(1024), 0, st, scan_part, n_scan, total);
+            hipLaunchKernelGGL(desc_scan_apply, dim3(n_scan), dim3(1024), 0, st, hist, n_keys,
+                               scan_part, cursor);
             hipLaunchKernelGGL(desc_scatter, dim3(rb), dim3(kDescThreads), 0, st, bm[cur], n_q, B,
                                n_cat, cursor, pairs, pkey);
             if (E == 8)
