@@ -916,27 +916,35 @@ finalize_kernel(Rows rows, const uint64_t* __restrict__ part, int n_splits, int 
     int nv = 0;
     while (nv < KP && key[nv] != 0) ++nv;
     const int64_t qnm = qn[row];
-    for (int j = 0; j < nv; ++j) {
-        const int idx = idx_of_key(key[j]);
-        const int64_t d = rows.dot(lane, row, idx);
-        if (lane == 0) { s_i[wv][j] = idx; s_c[wv][j] = cos64(d, qnm, rn[idx]); }
-    }
-    __builtin_amdgcn_wave_barrier();
-    if (lane != 0) return;
+    // Exact scores in list order (upper-bound keys, descending), kept
+    // sorted (c desc, index asc).  Once k are scored, a candidate whose key
+    // score plus the FP32 margin is below the k-th exact score cannot enter
+    // the top k, nor can any after it: the rest of the list is skipped.
     double* cs = s_c[wv];
     int* ix = s_i[wv];
-    for (int a = 1; a < nv; ++a) {  // insertion sort: c desc, index asc
-        const double c = cs[a];
-        const int i = ix[a];
-        int p = a;
-        while (p > 0 && (cs[p - 1] < c || (cs[p - 1] == c && ix[p - 1] > i))) {
-            cs[p] = cs[p - 1];
-            ix[p] = ix[p - 1];
-            --p;
+    int nc = 0;  // candidates scored
+    for (int j = 0; j < nv; ++j) {
+        if (j >= k) {  // every lane reads the same LDS values: uniform
+            const float u = f32_of_ord((uint32_t)(key[j] >> 32));
+            if ((double)u + (double)kMarginEps < cs[k - 1]) break;
         }
-        cs[p] = c;
-        ix[p] = i;
+        const int idx = idx_of_key(key[j]);
+        const int64_t d = rows.dot(lane, row, idx);
+        if (lane == 0) {
+            const double c = cos64(d, qnm, rn[idx]);
+            int p = j;
+            while (p > 0 && (cs[p - 1] < c || (cs[p - 1] == c && ix[p - 1] > idx))) {
+                cs[p] = cs[p - 1];
+                ix[p] = ix[p - 1];
+                --p;
+            }
+            cs[p] = c;
+            ix[p] = idx;
+        }
+        __builtin_amdgcn_wave_barrier();
+        nc = j + 1;
     }
+    if (lane != 0) return;
     bool flag = false;
     if (nv == KP && nr > KP) {
         const float s_kp = f32_of_ord((uint32_t)(key[KP - 1] >> 32));
@@ -944,7 +952,7 @@ finalize_kernel(Rows rows, const uint64_t* __restrict__ part, int n_splits, int 
     }
     for (int j = 0; j < k; ++j) {
         const int64_t o = (int64_t)row * k + j;
-        if (j < nv) {
+        if (j < nc) {
             out_idx[o] = ix[j];
             out_score[o] = (float)cs[j];
             if (out_cos) out_cos[o] = cs[j];
