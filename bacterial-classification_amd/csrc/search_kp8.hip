@@ -4,4 +4,5 @@
 namespace hvk {
 
 HVK_SEARCH_RUNS(8, )
+HVK_SEARCH_BOUNDED(8, )
 }  // namespace hvk
