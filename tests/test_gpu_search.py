@@ -182,3 +182,32 @@ def test_bipolar_single_limb():
     ref_idx, ref_cos = orc.topk16(q_np, db_np, 5)
     res = H.search_mfma(ctx, q, db, 5, q_two_limbs=False, r_two_limbs=False)
     _check(res, ref_idx, ref_cos, 5)
+
+
+def test_bounded_second_pass_close_siblings():
+    """Close siblings (0.2-1% divergence, 100 per ancestor): the bounded
+    form's 8-entry lists for top-5 fail the margin check for some reads, and
+    the exact second pass settles them.  Its results equal the two-limb
+    form's over all reads and the FP64 oracle's on a sample."""
+    import hdc_amd as H
+    import torch
+    spec = H.make_spec(seed=5, n_ancestors=8, desc_per_ancestor=100, anc_len_min=1300,
+                       anc_len_max=1600, div_min=0.002, div_max=0.01, n_reads=3000,
+                       read_len_min=150, read_len_max=300, err_min=0.005, err_max=0.02)
+    w = H.generate(spec)
+    ctx = H.Context(DIM, K)
+    db = H.encode(ctx, H.DeviceSeqs(w.refs, w.ref_offsets), clamp=True)
+    q = H.encode(ctx, H.DeviceSeqs(w.reads, w.read_offsets), clamp=True)
+    b = H.search_mfma_bounded(ctx, q, db, 5)
+    t = H.search_mfma(ctx, q, db, 5)
+    torch.cuda.synchronize()
+    n_res = int(b.n_rescored.item())
+    print(f"bounded second pass: {n_res} of {q.n} reads re-scored", flush=True)
+    assert n_res > 0
+    assert torch.equal(b.idx, t.idx)
+    assert torch.equal(b.cos64, t.cos64)
+    rc = db.counts.cpu().numpy()[:, :DIM]
+    qc = q.counts.cpu().numpy()[:256, :DIM]
+    ref_idx, ref_cos = orc.topk16(qc, rc, 5)
+    res = H.TopK(b.idx[:256], b.score[:256], b.cos64[:256], b.n_rescored)
+    _check(res, ref_idx, ref_cos, 5)
