@@ -5,10 +5,10 @@
 //
 // One level is six launches on the caller's stream, with no host round trip:
 //   1. bucket count: one key per live (read, beam entry) = its category
-//   2. bucket scan (one workgroup)
+//   2. bucket scan (tile sums, one-workgroup scan of the sums, tile scans)
 //   3. scatter: (read, entry) pairs grouped by category
 //   4. score: every pair against the candidate list of its category.  A
-//      wave takes 8 consecutive pairs of the grouped list (mostly one
+//      wave takes 4 consecutive pairs of the grouped list (mostly one
 //      category) and splits K across its lanes, so every load reads 1 KB of
 //      one row; each child slab is loaded once for all of the wave's pairs
 //      of that category.  v_dot2_i32_i16 multiplies int16 counts by the
@@ -243,7 +243,9 @@ __device__ __forceinline__ int32_t sdot2(int32_t a, int32_t b, int32_t c) {
 // of its pairs (wave-uniform), kDescG children at a time; each child slab is
 // loaded once for all the wave's pairs of that category.  The per-lane int64
 // sums are reduced across the wave at the end of each pass.
-constexpr int kDescP = 8;                 // pairs per wave
+// pairs per wave: 4 leaves ~120 VGPRs and four waves per SIMD, which beat
+// 8 pairs (one child load shared by more pairs, two waves per SIMD) and 2
+constexpr int kDescP = 4;
 constexpr int kDescStep = 64 * 8;         // elements per wave step
 
 __device__ __forceinline__ int64_t wave_sum_i64(int64_t v) {
