@@ -375,7 +375,9 @@ __device__ __forceinline__ uint32_t pack4(int a, int b, int c, int d) {
            ((uint32_t)(d & 0xFF) << 24);
 }
 
-__global__ void __launch_bounds__(256)
+// (Two workgroups per CU: the 160 accumulator registers and the rest fit in
+// 256, and the second wave per SIMD hides the read-row loads.)
+__global__ void __launch_bounds__(256, 2)
 desc_dense(TreeDev t, const int16_t* __restrict__ q, int64_t qld, const int64_t* __restrict__ qn,
            int n, int B, int64_t K, int Fmax, float* __restrict__ sc) {
     __shared__ __attribute__((aligned(16))) int8_t bl[4][kDenseN][kDenseRow];
