@@ -752,7 +752,9 @@ extern "C" int hvk_descend(hvk_ctx* ctx, const hvk_tree* tree, const int16_t* d_
     const DescOut out{d_path, d_path_score, d_path_len, d_exit_level};
     const int rb = (int)((n + kDescThreads - 1) / kDescThreads);
     const int pb = (int)((n * B + kDescP * (kDescThreads / 64) - 1) / (kDescP * (kDescThreads / 64)));
-    const int64_t K = std::min(q_ld, tld);
+    // columns [dim, ctx->ld) of every row are zero, and any valid stride is at
+    // least ctx->ld: the dots run over the context's ld, whatever the strides
+    const int64_t K = ctx->ld;
 
     // candidate lists of up to kSelMaxLds entries are sorted in LDS
     constexpr int kSelThreads = 128, kSelMaxLds = 64;
@@ -760,7 +762,8 @@ extern "C" int hvk_descend(hvk_ctx* ctx, const hvk_tree* tree, const int16_t* d_
     const size_t sel_bytes = sel_lds ? 2 * sizeof(float) * (size_t)maxN * kSelThreads : 0;
     // level 0 on the matrix cores when its operands fit the digit split
     const bool dense = tree->n_roots > 0 && tree->n_roots <= kDenseN && tree->max_abs_proto > 0 &&
-                       tree->max_abs_proto < (int64_t(1) << 30) && q_maxabs <= 32639;
+                       tree->max_abs_proto < (int64_t(1) << 30) && q_maxabs <= 32639 &&
+                       K < 65536;  // each digit-weight class sums < 2 K 2^14 in int32
     timing_mark(ctx, 1, false, st);
     hipLaunchKernelGGL(desc_init, dim3(rb), dim3(kDescThreads), 0, st, bm[0], n_q, B);
     HVK_HIP(hipGetLastError());

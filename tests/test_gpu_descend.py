@@ -192,27 +192,43 @@ def test_descend_ties_follow_std_sort():
 
 
 def test_descend_rejects_bad_trees():
+    """Host validation rejects malformed candidate lists and out-of-range limbs.
+    Every device argument is a real buffer of the right shape, so a validation
+    regression fails the status assertion instead of reading bad addresses."""
     import hdc_amd as H
     import ctypes
+    import torch
     ctx = H.Context(10240, 12)
+    n_cat, n_q, levels = 2, 4, 6
+    lo = torch.zeros((n_cat, ctx.ld), dtype=torch.int16, device="cuda")
+    hi = torch.zeros_like(lo)
+    sq = torch.ones(n_cat, dtype=torch.int64, device="cuda")
+    qc = torch.zeros((n_q, ctx.ld), dtype=torch.int16, device="cuda")
+    qn = torch.ones(n_q, dtype=torch.int64, device="cuda")
+    path = torch.zeros((n_q, levels), dtype=torch.int32, device="cuda")
+    score = torch.zeros((n_q, levels), dtype=torch.float32, device="cuda")
+    length = torch.zeros(n_q, dtype=torch.int32, device="cuda")
+    exit_level = torch.zeros(n_q, dtype=torch.int32, device="cuda")
     t = H.Tree()
-    t.n_cat = 2
+    t.n_cat = n_cat
     t.ld = ctx.ld
-    off = np.array([0, 1, 5], np.int32)  # second span runs past n entries of child_idx? (4 > 2)
+    off = np.array([0, 1, 5], np.int32)
     idx = np.array([1, 1, 7, 0, 0], np.int32)  # 7 is out of range
     roots = np.array([0], np.int32)
     t.child_off, t.child_idx, t.roots, t.n_roots = off.ctypes.data, idx.ctypes.data, \
         roots.ctypes.data, 1
-    t.d_lo = t.d_hi = t.d_sqnorm = 1  # never dereferenced: validation fails first
+    t.d_lo, t.d_hi, t.d_sqnorm = lo.data_ptr(), hi.data_ptr(), sq.data_ptr()
     t.max_abs_limb = 100
-    rc = H.lib().hvk_descend(ctx.h, ctypes.byref(t), 1, 1, 4, ctx.ld, 100, 3, 0.0, 6, 1, 1, 1, 1,
-                             None)
+    args = [H._ptr(qc), H._ptr(qn), n_q, ctx.ld, 100, 3, 0.0, levels, H._ptr(path), H._ptr(score),
+            H._ptr(length), H._ptr(exit_level), H._stream()]
+    rc = H.lib().hvk_descend(ctx.h, ctypes.byref(t), *args)
     assert rc == H.HVK_ERR_ARG
     idx[2] = 1
     t.max_abs_limb = 40000
-    rc = H.lib().hvk_descend(ctx.h, ctypes.byref(t), 1, 1, 4, ctx.ld, 100, 3, 0.0, 6, 1, 1, 1, 1,
-                             None)
+    rc = H.lib().hvk_descend(ctx.h, ctypes.byref(t), *args)
     assert rc == H.HVK_ERR_RANGE
+    torch.cuda.synchronize()
+    ctx.close()
 
 
 def test_dense_level0_equals_vector_path(synth):
