@@ -10,14 +10,22 @@
 // together.  The chunk's rows live in batch-owned buffers; after the search,
 // a scatter kernel writes each row's top-k to the read's own output row.
 //
-// Graph capture: the first call that meets a chunk with a given key (its
-// position and size, every input and output pointer, k and the settings)
-// runs the launches directly -- growing the private context's workspaces to
-// their final size -- and then captures the same launches on the batch's
-// stream into a hipGraph; later calls with the same key replay it.  The
-// private context is used by nothing else, so the workspace addresses the
-// graphs hold stay valid.  Nothing in a chunk uses hipMemsetAsync (counters
-// are cleared by kernels).
+// Graph capture.  The graphs touch only batch-owned buffers and the
+// database: each call first copies the caller's packed reads into the
+// batch's staging buffers, and after each chunk's graph a scatter launched
+// outside it writes the chunk's results to the caller's output rows.  So the
+// graphs do not depend on the caller's read or output buffers, and a stream
+// of calls with fresh buffers replays the same graphs.  A graph's key is the
+// chunk's position and size, k, the settings, the encoder's counter width
+// class of its longest read, the database pointers, and the addresses of the
+// batch's buffers and of the private context's workspaces (which only move
+// when they grow).  The first call that meets a key runs the chunk directly
+// -- growing every workspace to its final size -- and then captures the same
+// launches on the batch's stream into a hipGraph; later calls replay it.  At
+// most kMaxGraphs graphs are kept, the least recently used dropped first.
+// The private context is used by nothing else, so the workspace addresses
+// the graphs hold stay valid.  Nothing in a chunk uses hipMemsetAsync
+// (counters are cleared by kernels).
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -27,24 +35,20 @@
 namespace {
 
 constexpr int kMinChunk = 256;
+constexpr size_t kMaxGraphs = 64;
 
 struct GraphKey {
     int64_t c0;
     int n;
     int k;
     int form, splits;
-    int max_len;        // longest read of the chunk (selects the encoder's counter width)
-    const void* packed;
-    const void* invalid;
+    int width;          // counter width class of the chunk's longest read
     hvk_bundles db;
-    const void* idx;
-    const void* score;
-    const void* cos64;
-    const void* nres;
-    const void* order;  // the batch's buffers (move when they grow)
+    const void* in;     // the batch's buffers (move when they grow)
+    const void* order;
     const void* rows;
     const void* outs;
-    const void* ws[4];  // the private context's search workspaces
+    const void* ws[5];  // the private context's search workspaces
     bool operator==(const GraphKey& o) const { return std::memcmp(this, &o, sizeof(*this)) == 0; }
 };
 
@@ -62,6 +66,7 @@ struct hvk_batch {
     int chunk = 0;
     hipStream_t s = nullptr;
     hipEvent_t ev_in = nullptr, ev_out = nullptr, t0 = nullptr, t1 = nullptr;
+    hvk::Workspace ws_in;     // staging copy of the caller's packed reads
     hvk::Workspace ws_rows;   // chunk rows and per-row sums
     hvk::Workspace ws_out;    // chunk outputs
     hvk::Workspace ws_order;  // perm, starts, lens of the whole batch
@@ -139,11 +144,17 @@ int chunk_bufs(hvk_batch* b, int k, ChunkBufs& cb) {
     return HVK_OK;
 }
 
-// the launches of one chunk, on b->s
+// the encoder's counter width class of a chunk whose longest read has
+// max_len bases (launch_encode picks its bit-sliced counter width by it)
+int width_class(int max_len, int k) {
+    const int64_t nmax = (int64_t)max_len - k + 1;
+    return nmax < (1 << 11) ? 0 : (nmax < (1 << 16) ? 1 : 2);
+}
+
+// the launches of one chunk up to its results in cb, on b->s (captured)
 int run_chunk(hvk_batch* b, const uint32_t* packed, const uint32_t* invalid, int64_t c0, int n,
-              int max_len, const hvk_bundles& db, int k, const ChunkBufs& cb, const int32_t* perm,
-              const int64_t* starts, const int32_t* lens, int32_t* oi, float* os, double* oc,
-              int32_t* onres) {
+              int max_len, const hvk_bundles& db, int k, const ChunkBufs& cb,
+              const int64_t* starts, const int32_t* lens) {
     hvk_ctx* ctx = b->ctx;
     const int64_t ld = ctx->ld;
     EncodeOut eo{};
@@ -167,11 +178,28 @@ int run_chunk(hvk_batch* b, const uint32_t* packed, const uint32_t* invalid, int
         e = hvk_search_mfma(ctx, cb.lo, cb.hi, cb.sqnorm, n, db.lo, db.hi, db.sqnorm, db.n, ld, k,
                             cb.idx, cb.score, cb.cos64, cb.nres, b->s);
     }
-    if (e) return e;
+    return e;
+}
+
+// the chunk's results to the caller's rows (outside the graph)
+int scatter_chunk(hvk_batch* b, int64_t c0, int n, int k, const ChunkBufs& cb,
+                  const int32_t* perm, int32_t* oi, float* os, double* oc, int32_t* onres) {
     const int64_t tot = (int64_t)n * k;
     hipLaunchKernelGGL(batch_scatter, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, b->s,
                        perm + c0, n, k, cb.idx, cb.score, cb.cos64, oi, os, oc, cb.nres, onres);
     HVK_HIP(hipGetLastError());
+    return HVK_OK;
+}
+
+// least recently used graphs beyond kMaxGraphs are destroyed (after the
+// stream has finished any replay of them)
+int trim_graphs(hvk_batch* b) {
+    if (b->graphs.size() < kMaxGraphs) return HVK_OK;
+    HVK_HIP(hipStreamSynchronize(b->s));
+    while (b->graphs.size() >= kMaxGraphs) {
+        (void)hipGraphExecDestroy(b->graphs.front().exec);
+        b->graphs.erase(b->graphs.begin());
+    }
     return HVK_OK;
 }
 
@@ -219,13 +247,20 @@ int order_reads(hvk_batch* b, const int64_t* h_offsets, int n, int& max_len) {
     return HVK_OK;
 }
 
-int classify(hvk_batch* b, const uint32_t* packed, const uint32_t* invalid,
+int classify(hvk_batch* b, const uint32_t* caller_packed, const uint32_t* caller_invalid,
              const int64_t* h_offsets, int n, const hvk_bundles& db, int k, int32_t* oi,
              float* os, double* oc, int32_t* onres, hipStream_t caller) {
     HVK_HIP(hipEventRecord(b->ev_in, caller));
     HVK_HIP(hipStreamWaitEvent(b->s, b->ev_in, 0));
     int max_len = 0;
     if (int e = order_reads(b, h_offsets, n, max_len)) return e;
+    // stage the packed reads in batch-owned buffers (the graphs read these)
+    const size_t words = (size_t)hvk_pack_words(h_offsets[n]);
+    if (int e = b->ws_in.ensure(2 * al(words * 4))) return e;
+    uint32_t* packed = (uint32_t*)b->ws_in.ptr;
+    uint32_t* invalid = (uint32_t*)((char*)b->ws_in.ptr + al(words * 4));
+    HVK_HIP(hipMemcpyAsync(packed, caller_packed, words * 4, hipMemcpyDeviceToDevice, b->s));
+    HVK_HIP(hipMemcpyAsync(invalid, caller_invalid, words * 4, hipMemcpyDeviceToDevice, b->s));
     const char* op = (const char*)b->ws_order.ptr;
     const int32_t* perm = (const int32_t*)op;
     const int32_t* lens = (const int32_t*)(op + al((size_t)n * 4));
@@ -247,9 +282,7 @@ int classify(hvk_batch* b, const uint32_t* packed, const uint32_t* invalid,
             key.k = k;
             key.form = b->form;
             key.splits = b->splits;
-            key.max_len = chunk_max;
-            key.packed = packed;
-            key.invalid = invalid;
+            key.width = width_class(chunk_max, b->ctx->k);
             key.db.clamp8 = db.clamp8;  // field by field: the struct's padding stays zero
             key.db.lo = db.lo;
             key.db.hi = db.hi;
@@ -257,10 +290,7 @@ int classify(hvk_batch* b, const uint32_t* packed, const uint32_t* invalid,
             key.db.maxabs = db.maxabs;
             key.db.resid1 = db.resid1;
             key.db.n = db.n;
-            key.idx = oi;
-            key.score = os;
-            key.cos64 = oc;
-            key.nres = onres;
+            key.in = b->ws_in.ptr;
             key.order = b->ws_order.ptr;
             key.rows = b->ws_rows.ptr;
             key.outs = b->ws_out.ptr;
@@ -268,6 +298,7 @@ int classify(hvk_batch* b, const uint32_t* packed, const uint32_t* invalid,
             key.ws[1] = b->ctx->ws_norms.ptr;
             key.ws[2] = b->ctx->ws_flags.ptr;
             key.ws[3] = b->ctx->ws_search2.ptr;
+            key.ws[4] = b->ctx->ws_select.ptr;
             return key;
         };
         GraphKey key = make_key();
@@ -275,16 +306,18 @@ int classify(hvk_batch* b, const uint32_t* packed, const uint32_t* invalid,
                                [&](const Graph& g) { return g.key == key; });
         if (it != b->graphs.end()) {
             HVK_HIP(hipGraphLaunch(it->exec, b->s));
+            std::rotate(it, it + 1, b->graphs.end());  // most recently used last
+            if (int e = scatter_chunk(b, c0, m, k, cb, perm, oi, os, oc, onres)) return e;
             continue;
         }
         // direct run (sizes the workspaces), then capture the same launches
-        if (int e = run_chunk(b, packed, invalid, c0, m, chunk_max, db, k, cb, perm, starts, lens,
-                              oi, os, oc, onres))
+        if (int e = run_chunk(b, packed, invalid, c0, m, chunk_max, db, k, cb, starts, lens))
             return e;
+        if (int e = scatter_chunk(b, c0, m, k, cb, perm, oi, os, oc, onres)) return e;
         key = make_key();  // the run may have grown (moved) a workspace
+        if (int e = trim_graphs(b)) return e;
         HVK_HIP(hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal));
-        const int ce = run_chunk(b, packed, invalid, c0, m, chunk_max, db, k, cb, perm, starts,
-                                 lens, oi, os, oc, onres);
+        const int ce = run_chunk(b, packed, invalid, c0, m, chunk_max, db, k, cb, starts, lens);
         hipGraph_t g = nullptr;
         const hipError_t he = hipStreamEndCapture(b->s, &g);
         if (ce) {
@@ -306,7 +339,7 @@ int classify(hvk_batch* b, const uint32_t* packed, const uint32_t* invalid,
 int check_inputs(const hvk_batch* b, const uint32_t* packed, const uint32_t* invalid,
                  const int64_t* h_offsets, int n, const hvk_bundles* db, int k, const int32_t* oi,
                  const float* os) {
-    if (!b || n < 0 || !db || k < 1 || k > 29) return HVK_ERR_ARG;
+    if (!b || n < 0 || !db || k < 1) return HVK_ERR_ARG;
     if (n == 0) return HVK_OK;
     if (!packed || !invalid || !h_offsets || !oi || !os) return HVK_ERR_ARG;
     if (!db->clamp8 || !db->lo || !db->sqnorm || !db->maxabs || !db->resid1 || db->n <= 0)

@@ -13,6 +13,8 @@ void set_last_error(const char* where, hipError_t e) {
     g_last_error = std::string(where) + ": " + hipGetErrorString(e);
 }
 
+void set_last_message(const char* msg) { g_last_error = msg; }
+
 int Workspace::ensure(size_t need) {
     if (need <= bytes) return HVK_OK;
     if (ptr) {
@@ -149,6 +151,10 @@ int hvk_ctx_set_tuning(hvk_ctx* ctx, int param, int value) {
         case HVK_TUNE_SPLITS:
             if (value < 0 || value > 64) return HVK_ERR_ARG;
             ctx->tune_splits = value;
+            return HVK_OK;
+        case HVK_TUNE_SELECT_KIB:
+            if (value < 0) return HVK_ERR_ARG;
+            ctx->tune_select_kib = value;
             return HVK_OK;
         default:
             return HVK_ERR_ARG;

@@ -21,6 +21,7 @@ constexpr int kMaxTopK = 32;
 inline int64_t row_ld_for(int dim) { return (int64_t(dim) + 127) / 128 * 128; }
 
 void set_last_error(const char* where, hipError_t e);
+void set_last_message(const char* msg);
 
 #define HVK_HIP(call)                                   \
     do {                                                \
@@ -58,6 +59,7 @@ struct hvk_ctx {
     hvk::Workspace ws_search2;          // partial lists of the exact second pass
     hvk::Workspace ws_desc;             // hierarchical descent: beams, scores, buckets
     hvk::Workspace ws_desc_tree;        // hierarchical descent: uploaded candidate lists
+    hvk::Workspace ws_select;           // large-k selection: key matrix and sort buffers
     std::mutex mu;
     // kernel timing (hvk_ctx_set_timing)
     bool timing = false;
@@ -65,6 +67,7 @@ struct hvk_ctx {
     bool ev_used[2] = {false, false};  // encode, search
     // tuning (hvk_ctx_set_tuning); 0 = automatic
     int tune_splits = 0;
+    int tune_select_kib = 0;  // large-k selection workspace budget, 0 = 1 GiB
 };
 
 // Kernel launchers (defined in the .hip files).

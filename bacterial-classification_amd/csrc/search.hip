@@ -142,7 +142,7 @@ extern "C" int hvk_search_plain(hvk_ctx* ctx, const int16_t* d_q, const int64_t*
                                 int k, int32_t* d_idx, float* d_score, double* d_cos64,
                                 int32_t* d_n_rescored, hvk_stream stream) {
     if (!ctx || !d_q || !d_q_sqnorm || !d_r || !d_r_sqnorm || !d_idx || !d_score) return HVK_ERR_ARG;
-    if (n_q < 0 || n_r < 0 || k < 1 || k > 29 || q_maxabs < 0 || r_maxabs < 0) return HVK_ERR_ARG;
+    if (n_q < 0 || n_r < 0 || k < 1 || q_maxabs < 0 || r_maxabs < 0) return HVK_ERR_ARG;
     if (ld == 0) ld = ctx->ld;
     if (ld < ctx->dim || ld % hvk::kPlainKS) return HVK_ERR_ARG;
     if ((int64_t)q_maxabs * r_maxabs * hvk::kPlainKS >= (int64_t(1) << 31)) return HVK_ERR_RANGE;
@@ -150,6 +150,9 @@ extern "C" int hvk_search_plain(hvk_ctx* ctx, const int16_t* d_q, const int64_t*
     if (n_r == 0) return HVK_ERR_ARG;
     std::lock_guard<std::mutex> g(ctx->mu);
     hipStream_t st = (hipStream_t)stream;
+    if (k > hvk::kListMaxK)
+        return hvk::select_topk_plain(ctx, d_q, d_q_sqnorm, n_q, d_r, d_r_sqnorm, n_r, ld, k, d_idx,
+                                      d_score, d_cos64, d_n_rescored, st);
     switch (hvk::pick_kp(k)) {
         case 8: return hvk::run_plain<8>(ctx, d_q, d_q_sqnorm, n_q, d_r, d_r_sqnorm, n_r, ld, k, d_idx, d_score, d_cos64, d_n_rescored, st);
         case 16: return hvk::run_plain<16>(ctx, d_q, d_q_sqnorm, n_q, d_r, d_r_sqnorm, n_r, ld, k, d_idx, d_score, d_cos64, d_n_rescored, st);
@@ -164,7 +167,7 @@ extern "C" int hvk_search_mfma(hvk_ctx* ctx, const int8_t* d_q_lo, const int8_t*
                                double* d_cos64, int32_t* d_n_rescored, hvk_stream stream) {
     if (!ctx || !d_q_lo || !d_q_sqnorm || !d_r_lo || !d_r_sqnorm || !d_idx || !d_score)
         return HVK_ERR_ARG;
-    if (n_q < 0 || n_r < 0 || k < 1 || k > 29) return HVK_ERR_ARG;
+    if (n_q < 0 || n_r < 0 || k < 1) return HVK_ERR_ARG;
     if (ld == 0) ld = ctx->ld;
     if (ld < ctx->dim || ld % hvk::kMfmaKS) return HVK_ERR_ARG;
     if (ld > (int64_t(1) << 24)) return HVK_ERR_RANGE;  // tile offsets are 32-bit
@@ -172,6 +175,10 @@ extern "C" int hvk_search_mfma(hvk_ctx* ctx, const int8_t* d_q_lo, const int8_t*
     if (n_r == 0) return HVK_ERR_ARG;
     std::lock_guard<std::mutex> g(ctx->mu);
     hipStream_t st = (hipStream_t)stream;
+    if (k > hvk::kListMaxK)
+        return hvk::select_topk_limb(ctx, d_q_lo, d_q_hi, d_q_sqnorm, n_q, d_r_lo, d_r_hi,
+                                     d_r_sqnorm, n_r, ld, k, d_idx, d_score, d_cos64,
+                                     d_n_rescored, st);
     const int kp = hvk::pick_kp(k);
     const bool q2 = d_q_hi != nullptr, r2 = d_r_hi != nullptr;
     if (!q2 && !r2) return hvk::dispatch_mfma_kp<1, 1>(ctx, kp, d_q_lo, nullptr, d_q_sqnorm, n_q, d_r_lo, nullptr, d_r_sqnorm, n_r, ld, k, d_idx, d_score, d_cos64, d_n_rescored, st);
@@ -187,7 +194,7 @@ extern "C" int hvk_search_mfma_bounded(hvk_ctx* ctx, const hvk_bundles* q, const
     for (const hvk_bundles* x : {q, r})
         if (!x->clamp8 || !x->lo || !x->sqnorm || !x->maxabs || !x->resid1 || x->n < 0)
             return HVK_ERR_ARG;
-    if (k < 1 || k > 29) return HVK_ERR_ARG;
+    if (k < 1) return HVK_ERR_ARG;
     if (ld == 0) ld = ctx->ld;
     if (ld < ctx->dim || ld % hvk::kMfmaKS) return HVK_ERR_ARG;
     if (ld > (int64_t(1) << 24)) return HVK_ERR_RANGE;  // tile offsets are 32-bit
@@ -195,6 +202,9 @@ extern "C" int hvk_search_mfma_bounded(hvk_ctx* ctx, const hvk_bundles* q, const
     if (r->n == 0) return HVK_ERR_ARG;
     std::lock_guard<std::mutex> g(ctx->mu);
     hipStream_t st = (hipStream_t)stream;
+    if (k > hvk::kListMaxK)  // exact limbs, no bound needed
+        return hvk::select_topk_limb(ctx, q->lo, q->hi, q->sqnorm, q->n, r->lo, r->hi, r->sqnorm,
+                                     r->n, ld, k, d_idx, d_score, d_cos64, d_n_rescored, st);
     // The bounded keys are upper bounds, so a list twice as long keeps the
     // margin check conclusive for more queries; but the shorter list has
     // fewer offers in the search kernel, and finalize stops early, so it won

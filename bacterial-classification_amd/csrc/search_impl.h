@@ -57,6 +57,16 @@ __device__ __forceinline__ double cos64(int64_t dot, int64_t nq, int64_t nr) {
     return (double)dot / (sqrt((double)nq) * sqrt((double)nr));
 }
 
+// FP64 value <-> unsigned key of the same order (no NaN occurs here).
+__device__ __forceinline__ uint64_t ord_f64(double c) {
+    const uint64_t b = (uint64_t)__double_as_longlong(c);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double f64_of_ord(uint64_t o) {
+    const uint64_t b = (o >> 63) ? (o & 0x7FFFFFFFFFFFFFFFull) : ~o;
+    return __longlong_as_double((long long)b);
+}
+
 template <typename F, int... Is>
 __device__ __forceinline__ void static_for_impl(F&& f, std::integer_sequence<int, Is...>) {
     (f(std::integral_constant<int, Is>{}), ...);
@@ -167,11 +177,15 @@ struct BoundRows {  // nullable inputs of the bounded form
 // ------------------------------------------------------------------------
 constexpr int kPlainMT = 32, kPlainNT = 256, kPlainKS = 64;
 
-template <int KP>
+// MAT: instead of top lists, every pair's exact FP64 cosine, as an ord_f64
+// key, is written to mat[m][n] (row stride nr); qn64 / rn64 are the exact
+// squared norms.
+template <int KP, bool MAT = false>
 __global__ void __launch_bounds__(256)
 plain_kernel(const int16_t* __restrict__ q, const float* __restrict__ iq, int nq,
              const int16_t* __restrict__ r, const float* __restrict__ ir, int nr, int64_t ld,
-             int split_tiles, uint64_t* __restrict__ part) {
+             int split_tiles, uint64_t* __restrict__ part, const int64_t* __restrict__ qn64 = nullptr,
+             const int64_t* __restrict__ rn64 = nullptr, uint64_t* __restrict__ mat = nullptr) {
     constexpr int MT = kPlainMT, NT = kPlainNT, KS = kPlainKS, KW = KS / 2;
     constexpr int CAP = 16;
     __shared__ uint32_t sQ[KW][MT];
@@ -184,8 +198,10 @@ plain_kernel(const int16_t* __restrict__ q, const float* __restrict__ iq, int nq
     const int tx = tid & 31, ty = tid >> 5;
     const int m0 = blockIdx.x * MT;
     const int split = blockIdx.y;
-    tl.init(tid, 256);
-    if (tid < MT) s_iq[tid] = (m0 + tid < nq) ? iq[m0 + tid] : 0.0f;
+    if constexpr (!MAT) {
+        tl.init(tid, 256);
+        if (tid < MT) s_iq[tid] = (m0 + tid < nq) ? iq[m0 + tid] : 0.0f;
+    }
     const int n_begin = split * split_tiles * NT;
     int n_end = n_begin + split_tiles * NT;
     if (n_end > nr) n_end = nr;
@@ -249,6 +265,20 @@ plain_kernel(const int16_t* __restrict__ q, const float* __restrict__ iq, int nq
             __syncthreads();
         }
 
+        if constexpr (MAT) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int m = m0 + 4 * ty + i;
+                if (m >= nq) continue;
+                const int64_t qm = qn64[m];
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int n = n0 + tx + 32 * j;
+                    if (n < nr) mat[(int64_t)m * nr + n] = ord_f64(cos64(acc[i][j], qm, rn64[n]));
+                }
+            }
+            continue;
+        }
         float irv[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
@@ -271,9 +301,11 @@ plain_kernel(const int16_t* __restrict__ q, const float* __restrict__ iq, int nq
             if (!topk_pass_end<CAP>(tl, cbuf, tid, pass)) break;
         }
     }
-    for (int i = tid; i < MT * KP; i += 256) {
-        const int j = i / MT, m = i % MT;
-        if (m0 + m < nq) part[((int64_t)split * nq + m0 + m) * KP + j] = tl.top[j][m];
+    if constexpr (!MAT) {
+        for (int i = tid; i < MT * KP; i += 256) {
+            const int j = i / MT, m = i % MT;
+            if (m0 + m < nq) part[((int64_t)split * nq + m0 + m) * KP + j] = tl.top[j][m];
+        }
     }
 }
 
@@ -384,14 +416,20 @@ __device__ __forceinline__ void lds_barrier() { asm volatile("s_barrier" ::: "me
 // |dot - dot_clamped| * iq * ir.
 // IND: query m of the launch is stored row q_rows[m], and the launch covers
 // m < *q_count (a count on the device, at most nq).
-template <int QL, int RL, int KP, bool BND = false, bool IND = false>
+// MAT: no top lists; the exact dot of every pair (limb products combined in
+// int64) gives its FP64 cosine, written as an ord_f64 key to mat[m][n] (row
+// stride nr); qn64 / rn64 are the exact squared norms.
+template <int QL, int RL, int KP, bool BND = false, bool IND = false, bool MAT = false>
 __global__ void __launch_bounds__(kMfmaThreads, 1)
 mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
             const float* __restrict__ iq, int nq, const int8_t* __restrict__ rlo,
             const int8_t* __restrict__ rhi, const float* __restrict__ ir, int nr, int64_t ld,
             TileMap map, uint64_t* __restrict__ part, const float2* __restrict__ bq,
             const float2* __restrict__ br, const int32_t* __restrict__ q_rows,
-            const int32_t* __restrict__ q_count, uint64_t* __restrict__ qthr) {
+            const int32_t* __restrict__ q_count, uint64_t* __restrict__ qthr,
+            const int64_t* __restrict__ qn64 = nullptr, const int64_t* __restrict__ rn64 = nullptr,
+            uint64_t* __restrict__ mat = nullptr) {
+    static_assert(!(MAT && (IND || BND)), "the key matrix takes direct rows and exact limbs");
     // qthr (nullable, zeroed by the caller): per query, the largest
     // threshold key any slice has reached.  Every slice publishes its own
     // after each tile and adopts the largest: a slice's KP-th best key is a
@@ -427,11 +465,13 @@ mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
     const int wm = wave % kMfmaGM, wn = wave / kMfmaGM;
     auto qrow = [&](int m) { return IND ? q_rows[m] : m; };
-    tl.init(tid, NTH);
-    for (int i = tid; i < MT; i += NTH) {
-        const bool in = m0 + i < nq;
-        sm.iq[i] = in ? iq[qrow(m0 + i)] : 0.0f;
-        sm.bq[i] = (BND && in) ? bq[qrow(m0 + i)] : make_float2(0.0f, 0.0f);
+    if constexpr (!MAT) {
+        tl.init(tid, NTH);
+        for (int i = tid; i < MT; i += NTH) {
+            const bool in = m0 + i < nq;
+            sm.iq[i] = in ? iq[qrow(m0 + i)] : 0.0f;
+            sm.bq[i] = (BND && in) ? bq[qrow(m0 + i)] : make_float2(0.0f, 0.0f);
+        }
     }
     const int n_begin = split * map.split_tiles * NT;
     int n_end = n_begin + map.split_tiles * NT;
@@ -640,6 +680,38 @@ mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
         wait_vmcnt<0>();
         __syncthreads();
 
+        if constexpr (MAT) {
+            // pair e of block (x, y): query row mb(x) + 8(e/4) + e%4, reference
+            // column n0 + wn*32*WY + y*32 + lane%32 (the 32x32 output layout);
+            // the 32 lanes of a row write 256 consecutive bytes
+            const int lane_m = opaque_lane();
+            static_for<WY>([&](auto Y) {
+                constexpr int y = decltype(Y)::value;
+                const int n = n0 + wn * 32 * WY + y * 32 + (lane_m & 31);
+                if (n < nr) {
+                    const int64_t rnn = rn64[n];
+                    static_for<WX>([&](auto X) {
+                        constexpr int x = decltype(X)::value;
+                        const int mb = wm * 32 * WX + x * 32 + 4 * (lane_m >> 5);
+                        static_for<16>([&](auto E) {
+                            constexpr int e = decltype(E)::value;
+                            const int m = m0 + mb + 8 * (e >> 2) + (e & 3);
+                            if (m < nq) {
+                                int64_t d = 0;
+                                static_for<QL * RL>([&](auto J) {
+                                    constexpr int qa = decltype(J)::value / RL;
+                                    constexpr int rb = decltype(J)::value % RL;
+                                    d += (int64_t)acc[qa][rb][x][y][e] << (8 * (qa + rb));
+                                });
+                                mat[(int64_t)m * nr + n] = ord_f64(cos64(d, qn64[m], rnn));
+                            }
+                        });
+                    });
+                }
+            });
+            continue;
+        }
+
         // The epilogue derives its addresses from an opaque copy of the lane
         // index, so that none of them is computed ahead and kept live
         // through the K loop.
@@ -797,9 +869,11 @@ mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
             }
         }
     }
-    for (int i = tid; i < MT * KP; i += NTH) {
-        const int j = i / MT, m = i % MT;
-        if (m0 + m < nq) part[((int64_t)split * nq + m0 + m) * KP + j] = tl.top[j][m];
+    if constexpr (!MAT) {
+        for (int i = tid; i < MT * KP; i += NTH) {
+            const int j = i / MT, m = i % MT;
+            if (m0 + m < nq) part[((int64_t)split * nq + m0 + m) * KP + j] = tl.top[j][m];
+        }
     }
 }
 
@@ -1035,6 +1109,9 @@ rescan_kernel(Rows rows, const int64_t* __restrict__ qn, const int64_t* __restri
 // host side
 // ------------------------------------------------------------------------
 inline int pick_kp(int k) { return k <= 5 ? 8 : (k <= 13 ? 16 : 32); }
+// Largest k the candidate-list kernels serve (the margin check needs a list
+// longer than k); larger k go through the key-matrix selection (select.hip).
+constexpr int kListMaxK = 29;
 
 struct SearchBufs {
     float* iq;
@@ -1059,25 +1136,33 @@ constexpr int kPass2Cap = 16384;  // queries the exact second pass takes at once
 constexpr int kPass2Splits = 16;
 constexpr int kPass2KP = 32;
 
-inline int prepare(hvk_ctx* ctx, int nq, int nr, int n_splits, int kp, SearchBufs& b) {
+// Bytes of each search buffer for nq queries, nr references, S slices of
+// kp-entry lists; the exact second pass takes min(nq, kPass2Cap) queries.
+struct SearchSizes {
+    size_t norms, part, flags, part2;
+};
+inline SearchSizes search_sizes(int nq, int nr, int n_splits, int kp) {
     const size_t nqp = ((size_t)nq + 63) / 64 * 64, nrp = ((size_t)nr + 63) / 64 * 64;
-    const size_t need_n = sizeof(float) * (nqp + nrp) + sizeof(float2) * (nqp + nrp) +
-                          sizeof(uint64_t) * nqp;
-    int st = ctx->ws_norms.ensure(need_n);
-    if (st) return st;
-    st = ctx->ws_search.ensure(sizeof(uint64_t) * (size_t)n_splits * nq * kp);
-    if (st) return st;
-    st = ctx->ws_flags.ensure(sizeof(int32_t) * (4 * (size_t)nq + 2 * (size_t)nr + 512));
-    if (st) return st;
-    st = ctx->ws_search2.ensure(sizeof(uint64_t) * (size_t)kPass2Splits * kPass2Cap * kPass2KP);
-    if (st) return st;
-    b.iq = (float*)ctx->ws_norms.ptr;
+    const size_t cap2 = (size_t)(nq < kPass2Cap ? nq : kPass2Cap);
+    SearchSizes z;
+    z.norms = sizeof(float) * (nqp + nrp) + sizeof(float2) * (nqp + nrp) + sizeof(uint64_t) * nqp;
+    z.part = sizeof(uint64_t) * (size_t)n_splits * nq * kp;
+    z.flags = sizeof(int32_t) * (4 * (size_t)nq + 2 * (size_t)nr + 512);
+    z.part2 = sizeof(uint64_t) * (size_t)kPass2Splits * cap2 * kPass2KP;
+    return z;
+}
+
+// Points b into the four regions (sized by search_sizes).
+inline void carve(SearchBufs& b, void* norms, void* part, void* flags, void* part2, int nq,
+                  int nr) {
+    const size_t nqp = ((size_t)nq + 63) / 64 * 64, nrp = ((size_t)nr + 63) / 64 * 64;
+    b.iq = (float*)norms;
     b.ir = b.iq + nqp;
     b.bq = reinterpret_cast<float2*>(b.ir + nrp);
     b.br = b.bq + nqp;
     b.qthr = reinterpret_cast<uint64_t*>(b.br + nrp);
-    b.part = (uint64_t*)ctx->ws_search.ptr;
-    b.flag_count = (int32_t*)ctx->ws_flags.ptr;
+    b.part = (uint64_t*)part;
+    b.flag_count = (int32_t*)flags;
     b.flag_count2 = b.flag_count + 32;
     b.flag_list = b.flag_count + 64;
     b.flag_list2 = b.flag_list + nq + 64;
@@ -1086,7 +1171,17 @@ inline int prepare(hvk_ctx* ctx, int nq, int nr, int n_splits, int kp, SearchBuf
     b.ohist = b.order + nq;
     b.ocur = b.ohist + nr + 1;
     b.ototal = b.ocur + nr + 1;
-    b.part2 = (uint64_t*)ctx->ws_search2.ptr;
+    b.part2 = (uint64_t*)part2;
+}
+
+// The search buffers in the context's workspaces (grown on demand).
+inline int prepare(hvk_ctx* ctx, int nq, int nr, int n_splits, int kp, SearchBufs& b) {
+    const SearchSizes z = search_sizes(nq, nr, n_splits, kp);
+    if (int st = ctx->ws_norms.ensure(z.norms)) return st;
+    if (int st = ctx->ws_search.ensure(z.part)) return st;
+    if (int st = ctx->ws_flags.ensure(z.flags)) return st;
+    if (int st = ctx->ws_search2.ensure(z.part2)) return st;
+    carve(b, ctx->ws_norms.ptr, ctx->ws_search.ptr, ctx->ws_flags.ptr, ctx->ws_search2.ptr, nq, nr);
     return HVK_OK;
 }
 
@@ -1166,6 +1261,18 @@ int rescan(hvk_ctx* ctx, const RowsI16& rows, const int32_t* list, const int32_t
 void launch_inv_norm(const int64_t* qn, int nq, const int64_t* rn, int nr, const SearchBufs& b,
                      const BoundRows& bnd, bool bounds, hipStream_t st);
 
+// Exact top-k for kListMaxK < k (any k; k > nr pads with -1): the key
+// matrix of a chunk of queries from the plain kernel (int16 rows) or the
+// MFMA kernel (exact limbs; hi rows nullable), then a radix select and sort
+// per query.  *n_res (nullable) is set to 0: nothing is re-scored.
+int select_topk_plain(hvk_ctx* ctx, const int16_t* q, const int64_t* qn, int nq, const int16_t* r,
+                      const int64_t* rn, int nr, int64_t ld, int k, int32_t* idx, float* score,
+                      double* cosv, int32_t* n_res, hipStream_t st);
+int select_topk_limb(hvk_ctx* ctx, const int8_t* qlo, const int8_t* qhi, const int64_t* qn, int nq,
+                     const int8_t* rlo, const int8_t* rhi, const int64_t* rn, int nr, int64_t ld,
+                     int k, int32_t* idx, float* score, double* cosv, int32_t* n_res,
+                     hipStream_t st);
+
 // Exact second pass of the bounded form over the queries its first
 // finalize queued, then the FP64 re-scan of what remains.
 int exact_pass2(hvk_ctx* ctx, const hvk_bundles& q, const hvk_bundles& r, int64_t ld, int k,
@@ -1228,15 +1335,19 @@ int run_mfma(hvk_ctx* ctx, const int8_t* qlo, const int8_t* qhi, const int64_t* 
                                 b, pl.map.n_splits, qn, nq, rn, nr, k, idx, score, cosv, n_res, st);
 }
 
-template <int KP>
-int run_mfma_bounded(hvk_ctx* ctx, const hvk_bundles& q, const hvk_bundles& r, int64_t ld, int k,
-                     int32_t* idx, float* score, double* cosv, int32_t* n_res, hipStream_t st) {
-    const int nq = q.n, nr = r.n;
+// Reference slices and tile map of a bounded one-limb search.
+inline MfmaPlan plan_bounded(hvk_ctx* ctx, int nq, int nr) {
     using T = MfmaTile<1, 1>;
-    const MfmaPlan pl = plan_mfma(ctx, (nq + T::MT - 1) / T::MT, (nr + T::NT - 1) / T::NT, 0);
-    SearchBufs b;
-    int rc_ = prepare(ctx, nq, nr, pl.map.n_splits, KP, b);
-    if (rc_) return rc_;
+    return plan_mfma(ctx, (nq + T::MT - 1) / T::MT, (nr + T::NT - 1) / T::NT, 0);
+}
+
+// The launches of a bounded search on buffers already laid out (b sized by
+// search_sizes(nq, nr, pl.map.n_splits, KP)).
+template <int KP>
+int launch_mfma_bounded(hvk_ctx* ctx, const MfmaPlan& pl, const hvk_bundles& q,
+                        const hvk_bundles& r, int64_t ld, int k, const SearchBufs& b, int32_t* idx,
+                        float* score, double* cosv, int32_t* n_res, hipStream_t st) {
+    const int nq = q.n, nr = r.n;
     launch_inv_norm(q.sqnorm, nq, r.sqnorm, nr, b,
                     BoundRows{q.maxabs, q.resid1, r.maxabs, r.resid1}, true, st);
     timing_mark(ctx, 1, false, st);
@@ -1250,6 +1361,15 @@ int run_mfma_bounded(hvk_ctx* ctx, const hvk_bundles& q, const hvk_bundles& r, i
                                         k, idx, score, cosv, st, true))
         return rc;
     return exact_pass2(ctx, q, r, ld, k, b, idx, score, cosv, n_res, st);
+}
+
+template <int KP>
+int run_mfma_bounded(hvk_ctx* ctx, const hvk_bundles& q, const hvk_bundles& r, int64_t ld, int k,
+                     int32_t* idx, float* score, double* cosv, int32_t* n_res, hipStream_t st) {
+    const MfmaPlan pl = plan_bounded(ctx, q.n, r.n);
+    SearchBufs b;
+    if (int rc = prepare(ctx, q.n, r.n, pl.map.n_splits, KP, b)) return rc;
+    return launch_mfma_bounded<KP>(ctx, pl, q, r, ld, k, b, idx, score, cosv, n_res, st);
 }
 
 template <int QL, int RL>
@@ -1279,7 +1399,10 @@ int dispatch_mfma_kp(hvk_ctx* ctx, int kp, const int8_t* qlo, const int8_t* qhi,
 #define HVK_SEARCH_BOUNDED(KP, EXT)                                                               \
     EXT template int run_mfma_bounded<KP>(hvk_ctx*, const hvk_bundles&, const hvk_bundles&,       \
                                           int64_t, int, int32_t*, float*, double*, int32_t*,      \
-                                          hipStream_t);
+                                          hipStream_t);                                            \
+    EXT template int launch_mfma_bounded<KP>(hvk_ctx*, const MfmaPlan&, const hvk_bundles&,       \
+                                             const hvk_bundles&, int64_t, int, const SearchBufs&, \
+                                             int32_t*, float*, double*, int32_t*, hipStream_t);
 #define HVK_RUN_MFMA_ARGS                                                                          \
     hvk_ctx*, const int8_t*, const int8_t*, const int64_t*, int, const int8_t*, const int8_t*,    \
         const int64_t*, int, int64_t, int, int32_t*, float*, double*, int32_t*, hipStream_t

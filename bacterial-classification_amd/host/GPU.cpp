@@ -39,7 +39,6 @@ std::unordered_map<HV_16*, std::vector<std::pair<float, HV_16*>>> cosine_search(
         for (HV_16* p : inputs) result[p];
         return result;
     }
-    if (k > 29) throw std::runtime_error("cosine_search: k > 29 is not supported");
     auto& s = hvk_host::Session::instance();
     const hvk_host::DeviceBundles q = s.upload_counts({inputs.begin(), inputs.end()});
     const hvk_host::DeviceBundles r = s.upload_counts({db.begin(), db.end()});

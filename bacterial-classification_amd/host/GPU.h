@@ -21,6 +21,7 @@ std::vector<std::vector<HV_16>> encode(const std::vector<int16_t>& kmer_map,
 // Top-k cosine search (src/cosine.cu:502-564): for each input, the k (clamped
 // to db.size()) most similar db entries as (cosine, db pointer), cosine
 // descending, ties by db position; cosine = dot / (|a| |b|) (the GPU scale,
-// not the 2c - 1 of cosine() in HV.h), 0 for a zero vector.  k <= 29.
+// not the 2c - 1 of cosine() in HV.h), 0 for a zero vector.  Any k: past
+// the candidate-list length the library selects from the exact key matrix.
 std::unordered_map<HV_16*, std::vector<std::pair<float, HV_16*>>> cosine_search(
     const std::vector<HV_16*>& inputs, const std::vector<HV_16*>& db, int k = 1);

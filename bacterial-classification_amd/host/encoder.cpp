@@ -172,16 +172,22 @@ SIM HVCache::similarity(const std::string& l, const std::string& r) {
     // Candidates come ordered by (c desc, index asc); distinct c may round to
     // the same float, so take the earliest window among those whose float
     // equals the best one.
-    const int kk = (int)std::min<size_t>(wins.size(), 29);
-    const hvk_host::Session::TopK t = s.search(qd, wd, kk);
+    int kk = (int)std::min<size_t>(wins.size(), 29);
+    hvk_host::Session::TopK t = s.search(qd, wd, kk);
     const float best = two_c_minus_one(t.cos[0]);
-    size_t first = (size_t)t.idx[0];
-    int j = 1;
-    for (; j < kk && two_c_minus_one(t.cos[j]) == best; ++j) first = std::min(first, (size_t)t.idx[j]);
+    auto earliest = [&](int n, int& j) {
+        size_t first = (size_t)t.idx[0];
+        for (j = 1; j < n && two_c_minus_one(t.cos[j]) == best; ++j)
+            first = std::min(first, (size_t)t.idx[j]);
+        return first;
+    };
+    int j = 0;
+    size_t first = earliest(kk, j);
     if (j == kk && kk < (int)wins.size()) {
-        // the float tie runs past the candidate list: scan every window
-        for (size_t w = 0; w < wins.size(); ++w)
-            if (cosine(q, wins[w]) == best) { first = w; break; }
+        // the float tie runs past the candidate list: rank every window
+        kk = (int)wins.size();
+        t = s.search(qd, wd, kk);
+        first = earliest(kk, j);
     }
     if (best > sim.score) {
         sim.score = best;

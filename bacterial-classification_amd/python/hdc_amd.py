@@ -41,6 +41,7 @@ BATCH_FORM_BOUNDED = 0
 BATCH_FORM_TWO_LIMB = 1
 
 TUNE_SPLITS = 1
+TUNE_SELECT_KIB = 2
 
 
 class HvkError(RuntimeError):

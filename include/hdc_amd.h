@@ -70,11 +70,14 @@ int hvk_base_hv(const hvk_ctx* ctx, char symbol, uint64_t* host_out);
 int hvk_ctx_set_timing(hvk_ctx* ctx, int enable);
 int hvk_ctx_last_timing(hvk_ctx* ctx, float* encode_ms, float* search_ms);
 
-/* Tuning of the MFMA search launch (no counterpart in the reference; used by
- * the tile autotuner).  HVK_TUNE_SPLITS: number of reference slices per
- * query tile, 0 = automatic (the default), else 1..64.  Results do not
- * depend on it. */
+/* Tuning of the search launches (no counterpart in the reference; used by
+ * the tile autotuner and the tests).  Results do not depend on it.
+ * HVK_TUNE_SPLITS: number of reference slices per query tile of the MFMA
+ * search, 0 = automatic (the default), else 1..64.
+ * HVK_TUNE_SELECT_KIB: workspace budget (KiB) of the selection path for
+ * k > 29, which bounds how many queries one chunk holds; 0 = 1 GiB. */
 #define HVK_TUNE_SPLITS 1
+#define HVK_TUNE_SELECT_KIB 2
 int hvk_ctx_set_tuning(hvk_ctx* ctx, int param, int value);
 
 /* ---- packing (host) -----------------------------------------------------
@@ -168,7 +171,12 @@ int hvk_prepare_counts(hvk_ctx* ctx, const int16_t* d_counts, int n, const hvk_e
  *   the reference's cosine_search, src/cosine.cu:270), d_cos64 (nullable)
  *   double c.
  * d_n_rescored (nullable, int32 on the device): count of queries that took
- * the exact re-scan.  k <= 32.
+ * the exact re-scan.  Any k >= 1, as the reference's launch_cosine_topk
+ * (src/cosine.cu:450, which clamps k to num_db): for k <= 29 the search
+ * kernels keep per-query candidate lists; above, every pair's exact cosine
+ * goes to a key matrix (chunks of queries, at most 1 GiB of workspace) and
+ * a radix select and sort per query give the top k (nothing re-scored).
+ * Rows past n_r hold -1 / -inf.
  *
  * Plain form (VALU int16 dot): rows of int16 counts with stride ld (0 ->
  * ctx row_ld).  Requires max|q| * max|r| * 64 < 2^31; the caller passes the
@@ -327,10 +335,14 @@ int hvk_db_upload(const hvk_db* db, const hvk_ctx* ctx, const hvk_encode_out* ds
  *  - reads are ordered by length, longest first, and cut into chunks of at
  *    most `chunk` reads, so every encoder launch holds reads of similar
  *    length (its workgroups finish together);
- *  - a chunk runs encode -> MFMA search -> finalize -> scatter of its rows'
- *    results to the reads' own rows.  The first call that meets a chunk
- *    (same position, size, buffers and settings) runs it directly and
- *    captures the same launches into a hipGraph; later calls replay it;
+ *  - each call copies the packed reads into batch-owned staging buffers; a
+ *    chunk runs encode -> MFMA search -> finalize on batch-owned buffers,
+ *    then a scatter (outside the graph) writes its rows' results to the
+ *    reads' own output rows.  The first call that meets a chunk (same
+ *    position, size, k, settings, counter width of its longest read and
+ *    database) runs it directly and captures the same launches into a
+ *    hipGraph; later calls replay it, whatever their read and output
+ *    buffers.  At most 64 graphs are kept (least recently used dropped);
  *  - hvk_batch_autotune times candidate settings (search form: bounded one
  *    limb on 256x256 tiles or two limbs on 256x128 / 128x128 tiles;
  *    reference slices per read tile; chunk size) on a sample of the batch and
@@ -351,7 +363,8 @@ int hvk_batch_get_config(const hvk_batch* b, int* form, int* splits, int* chunk,
 /* d_packed / d_invalid: hvk_pack layout of all reads (device); h_offsets:
  * n+1 base offsets (host).  Outputs [n][k] (device); d_cos64 and
  * d_n_rescored (the number of reads re-scored by the exact passes, summed
- * over chunks) are nullable.  k: 1..16.  Enqueued on `stream`. */
+ * over chunks) are nullable.  Any k >= 1 (see hvk_search_plain).  Enqueued
+ * on `stream`. */
 int hvk_batch_classify(hvk_batch* b, const uint32_t* d_packed, const uint32_t* d_invalid,
                        const int64_t* h_offsets, int n, const hvk_bundles* db, int k,
                        int32_t* d_idx, float* d_score, double* d_cos64, int32_t* d_n_rescored,

@@ -16,6 +16,7 @@ import oracle_py as orc
 pytestmark = pytest.mark.gpu
 
 DIM, K, TOPK = 10240, 12, 5
+_HOST_READS = []  # the fixture's reads, for tests that upload them again
 
 
 @pytest.fixture(scope="module")
@@ -29,6 +30,7 @@ def work():
     reads = [w.reads[w.read_offsets[i]:w.read_offsets[i + 1]] for i in range(len(w.read_offsets) - 1)]
     reads += [b"", b"ACGTACGTAC", b"N" * 400, reads[3].lower()]
     buf, off = H.concat([r.decode() for r in reads])
+    _HOST_READS[:] = reads
     ctx = H.Context(DIM, K)
     db = H.encode(ctx, H.DeviceSeqs(w.refs, w.ref_offsets), counts=True, limbs=True, hi=True,
                   clamp=True)
@@ -81,20 +83,50 @@ def test_batch_equals_direct(work, chunk, form, splits):
 
 
 def test_batch_new_outputs_and_k(work):
-    """Other output buffers or another k are other graphs; results stay exact."""
+    """Fresh output buffers replay the same graphs (the graphs touch only
+    batch-owned buffers); another k is other graphs; results stay exact."""
     import hdc_amd as H
     ctx, db, seqs, q, direct = work
     b = H.Batch(DIM, K, 512)
     dbb = H.bundles(db)
     o1 = b.classify(seqs, dbb, TOPK)
-    o2 = b.classify(seqs, dbb, TOPK)  # fresh buffers: new keys
+    n_graphs = b.get_config()["graphs"]
+    o2 = b.classify(seqs, dbb, TOPK)  # fresh output buffers: same graphs
     _same(o1, direct)
     _same(o2, direct)
+    assert b.get_config()["graphs"] == n_graphs
     d10 = H.search_mfma_bounded(ctx, q, db, 10)
     o3 = b.classify(seqs, dbb, 10)
     _same(o3, d10)
+    d40 = H.search_mfma_bounded(ctx, q, db, 40)  # past the candidate lists
+    o4 = b.classify(seqs, dbb, 40)
+    _same(o4, d40)
     o1 = b.classify(seqs, dbb, TOPK, out=o1)  # replay of the first graphs
     _same(o1, direct)
+    b.close()
+
+
+def test_batch_streaming_fresh_buffers_bounded_graphs(work):
+    """A stream of calls, each with freshly uploaded reads (a copy of the same
+    sequences, then a permutation of them) and fresh outputs: the graph count
+    stays at the number of chunks and every result equals the direct path."""
+    import hdc_amd as H
+    import torch
+    ctx, db, seqs, q, direct = work
+    b = H.Batch(DIM, K, 512)
+    dbb = H.bundles(db)
+    n_chunks = (seqs.n + 511) // 512
+    raw = list(_HOST_READS)
+    for rep in range(3):
+        order = np.arange(len(raw)) if rep == 0 else np.random.default_rng(rep).permutation(len(raw))
+        buf, off = H.concat([raw[i].decode() for i in order])
+        fresh = H.DeviceSeqs(buf, off)
+        out = b.classify(fresh, dbb, TOPK)
+        torch.cuda.synchronize()
+        inv = torch.from_numpy(np.argsort(order)).cuda()
+        assert torch.equal(out.idx[inv], direct.idx)
+        assert torch.equal(out.cos64[inv], direct.cos64)
+        assert b.get_config()["graphs"] <= 2 * n_chunks
     b.close()
 
 

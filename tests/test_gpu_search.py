@@ -211,3 +211,77 @@ def test_bounded_second_pass_close_siblings():
     ref_idx, ref_cos = orc.topk16(qc, rc, 5)
     res = H.TopK(b.idx[:256], b.score[:256], b.cos64[:256], b.n_rescored)
     _check(res, ref_idx, ref_cos, 5)
+
+
+# ---- k past the candidate lists: key matrix + radix select (any k) ----
+
+@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("k", [30, 64, 300, 357])
+def test_large_k_matches_oracle(workload, form, k):
+    """k above the list length (29) up to and past the number of references
+    (300 here): identical indices and FP64 values, -1 past the references."""
+    ctx, refs, reads, rc, qc, w = workload
+    print(f"large-k form={form} k={k} nq={reads.n} nr={refs.n}", flush=True)
+    ref_idx, ref_cos = orc.topk16(qc, rc, k)
+    res = run_form(form, ctx, reads, refs, k)
+    _check(res, ref_idx, ref_cos, k)
+    assert int(res.n_rescored.cpu().item()) == 0
+    if k > refs.n:
+        assert (res.idx.cpu().numpy()[:, refs.n:] == -1).all()
+
+
+@pytest.fixture(scope="module")
+def big_db():
+    """1,100 references (44 ancestors x 25) and 48 reads, for k = 1024."""
+    import hdc_amd as H
+    import torch
+    spec = H.make_spec(seed=11, n_ancestors=44, desc_per_ancestor=25, anc_len_min=1300,
+                       anc_len_max=1600, div_min=0.02, div_max=0.08, n_reads=48,
+                       read_len_min=150, read_len_max=300, err_min=0.005, err_max=0.02)
+    w = H.generate(spec)
+    ctx = H.Context(DIM, K)
+    refs = H.encode(ctx, H.DeviceSeqs(w.refs, w.ref_offsets), clamp=True)
+    reads = H.encode(ctx, H.DeviceSeqs(w.reads, w.read_offsets), clamp=True)
+    torch.cuda.synchronize()
+    return ctx, refs, reads, refs.counts.cpu().numpy()[:, :DIM], reads.counts.cpu().numpy()[:, :DIM]
+
+
+@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("k", [1024, 1100])
+def test_large_k_1024_and_all(big_db, form, k):
+    ctx, refs, reads, rc, qc = big_db
+    print(f"large-k form={form} k={k} nq={reads.n} nr={refs.n}", flush=True)
+    ref_idx, ref_cos = orc.topk16(qc, rc, k)
+    res = run_form(form, ctx, reads, refs, k)
+    _check(res, ref_idx, ref_cos, k)
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_large_k_ties_and_zero_norms(workload, form):
+    """40 identical references tie exactly (index order), a zero row scores
+    c = 0, and a zero query ranks every reference at c = 0 (index order)."""
+    ctx, refs, reads, rc, qc, w = workload
+    big = np.concatenate([rc[:30], np.repeat(rc[5:6], 40, axis=0), rc[30:60]])
+    big[77] = 0
+    qn = np.concatenate([rc[5:6], qc[:7], np.zeros((1, DIM), np.int16)])
+    for k in (45, 100):
+        ref_idx, ref_cos = orc.topk16(qn, big, k)
+        res = run_form(form, ctx, encoded_from_counts(qn, ctx.ld), encoded_from_counts(big, ctx.ld),
+                       k)
+        _check(res, ref_idx, ref_cos, k)
+        assert list(res.idx.cpu().numpy()[8]) == list(range(k))
+        assert list(ref_idx[0][:41]) == [5] + list(range(30, 70))
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_large_k_chunked(workload, form):
+    """A 256 KiB selection budget splits the 157 queries into several chunks."""
+    import hdc_amd as H
+    ctx, refs, reads, rc, qc, w = workload
+    ref_idx, ref_cos = orc.topk16(qc, rc, 64)
+    ctx.set_tuning(H.TUNE_SELECT_KIB, 256)
+    try:
+        res = run_form(form, ctx, reads, refs, 64)
+        _check(res, ref_idx, ref_cos, 64)
+    finally:
+        ctx.set_tuning(H.TUNE_SELECT_KIB, 0)
