@@ -114,6 +114,11 @@ int hvk_ctx_create(int dim, int kmer, hvk_ctx** out) {
         delete c;
         return HVK_ERR_HIP;
     }
+    if (int rc = hvk::noalloc_setup(dev)) {
+        (void)hipFree(c->d_table);
+        delete c;
+        return rc;
+    }
     *out = c;
     return HVK_OK;
 }

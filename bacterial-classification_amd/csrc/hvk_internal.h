@@ -22,6 +22,9 @@ inline int64_t row_ld_for(int dim) { return (int64_t(dim) + 127) / 128 * 128; }
 
 void set_last_error(const char* where, hipError_t e);
 void set_last_message(const char* msg);
+// Resolves, on `device` (current), the address of the allocation-free
+// search's scratch array (noalloc.hip); called when a context is created.
+int noalloc_setup(int device);
 
 #define HVK_HIP(call)                                   \
     do {                                                \

@@ -33,6 +33,7 @@ EXPORTS = [
     "hvk_batch_create", "hvk_batch_destroy", "hvk_batch_config", "hvk_batch_get_config",
     "hvk_batch_classify", "hvk_batch_autotune",
     "hvk_db_write", "hvk_db_open", "hvk_db_close", "hvk_db_info", "hvk_db_array", "hvk_db_upload",
+    "hvk_search_mfma_bounded_noalloc",
 ]
 
 DB_COUNTS, DB_LO, DB_HI, DB_CLAMP8, DB_SUMS = 1, 2, 4, 8, 16
@@ -123,6 +124,7 @@ def lib():
     L.hvk_encode_windows.argtypes = [vp, vp, vp, vp, i32, i32, ctypes.POINTER(EncodeOut), vp]
     L.hvk_search_mfma_bounded.argtypes = [vp, ctypes.POINTER(Bundles), ctypes.POINTER(Bundles),
                                           i64, i32, vp, vp, vp, vp, vp]
+    L.hvk_search_mfma_bounded_noalloc.argtypes = L.hvk_search_mfma_bounded.argtypes
     L.hvk_prepare_counts.argtypes = [vp, vp, i32, ctypes.POINTER(EncodeOut), vp]
     L.hvk_ctx_set_timing.argtypes = [vp, i32]
     L.hvk_ctx_set_tuning.argtypes = [vp, i32, i32]

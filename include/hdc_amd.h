@@ -225,6 +225,20 @@ int hvk_search_mfma_bounded(hvk_ctx* ctx, const hvk_bundles* q, const hvk_bundle
                             int k, int32_t* d_idx, float* d_score, double* d_cos64,
                             int32_t* d_n_rescored, hvk_stream stream);
 
+/* Allocation-free variant of hvk_search_mfma_bounded (no counterpart in the
+ * reference).  For callers that cannot allocate device memory: code under
+ * hipGraph capture, serving loops and harnesses that forbid hipMalloc.  Same
+ * arguments and results, k <= 29; the search buffers live in a fixed device
+ * array of HVK_NOALLOC_SCRATCH_BYTES that each GPU holds until the process
+ * ends, so no call allocates or frees.  Sizes whose buffers do not fit return
+ * HVK_ERR_RANGE (about 4096 queries x 262,144 references at k <= 29 fit).
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (other streams or threads) race and must be serialised. */
+#define HVK_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int hvk_search_mfma_bounded_noalloc(hvk_ctx* ctx, const hvk_bundles* q, const hvk_bundles* r,
+                                    int64_t ld, int k, int32_t* d_idx, float* d_score,
+                                    double* d_cos64, int32_t* d_n_rescored, hvk_stream stream);
+
 /* ---- hierarchical descent ----------------------------------------------
  * identify's rank-by-rank beam (targets/identify.cpp:89-202) over a taxonomy
  * whose categories each carry one integer prototype bundle (a leaf's is its
