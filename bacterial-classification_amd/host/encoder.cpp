@@ -30,7 +30,7 @@ float two_c_minus_one(double c) { return (float)(2.0 * (c - 0.5)); }
 }  // namespace
 
 HVCache::HVCache(size_t max_entries, size_t channel_) : capacity(max_entries), channel(channel_) {
-    if (channel > 8) return;
+    if (channel > 10) return;
     // every k-mer s maps to the HV of min(s, revcomp(s)) bound by mult
     const uint64_t total = 1ull << (2 * channel);
     std::string s(channel, 'A'), rc(channel, 'A');

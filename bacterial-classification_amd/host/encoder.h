@@ -22,8 +22,11 @@ struct SIM {
 
 class HVCache {
 public:
-    // kmers is filled when 4^channel <= 65536 (the reference always fills it;
-    // at k = 12 it would take ~23 GB); the GPU never reads it.
+    // kmers holds every k-mer over ACGT and its bound HV, as the reference's
+    // table (src/encoder.cpp:47-74), for channel <= 10 (4^10 entries, ~1.3 GB
+    // at D = 10000).  Above, it stays empty: the reference's table would take
+    // ~5.4 GB at k = 11 and ~23 GB at k = 12, and nothing here reads it (the
+    // GPU binds k-mers from the base HVs).
     explicit HVCache(size_t max_entries = CACHE_SIZE, size_t channel = KMER_SIZE);
 
     const HV_16& get(const std::string& seq);

@@ -80,6 +80,7 @@ def reference():
         L.ref_precompute_similarity.argtypes = [ctypes.c_char_p, sz, ctypes.c_char_p, sz,
                                                 ctypes.POINTER(sz)]
         L.ref_precompute_windows.argtypes = [ctypes.c_char_p, sz, vp, ctypes.c_int]
+        L.ref_kmers_all.argtypes = [vp]
         _r = L
     return _r
 

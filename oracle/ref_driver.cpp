@@ -56,6 +56,25 @@ int ref_kmer_hv(const char* kmer, uint64_t* out) {
     return 1;
 }
 
+// The whole k-mer table of a fresh HVCache: its entry count, then the HV of
+// every k-mer over ACGT in lexicographic order (4^KMER_SIZE rows; a k-mer the
+// table lacks gets an all-zero row and is not counted).
+int ref_kmers_all(uint64_t* out) {
+    HVCache c(16);
+    const char acgt[4] = {'A', 'C', 'G', 'T'};
+    const size_t n = size_t(1) << (2 * KMER_SIZE);
+    std::string s(KMER_SIZE, 'A');
+    for (size_t i = 0; i < n; ++i) {
+        for (size_t p = 0; p < KMER_SIZE; ++p) s[p] = acgt[(i >> (2 * (KMER_SIZE - 1 - p))) & 3];
+        auto it = c.kmers.find(s);
+        if (it == c.kmers.end())
+            std::memset(out + i * HV_WORDS, 0, sizeof(uint64_t) * HV_WORDS);
+        else
+            std::memcpy(out + i * HV_WORDS, it->second.data(), sizeof(uint64_t) * HV_WORDS);
+    }
+    return (int)c.kmers.size();
+}
+
 void ref_compute(const char* seq, size_t len, int16_t* out) {
     const HV_16& h = compute_cache().get(std::string(seq, len));
     std::memcpy(out, h.data(), sizeof(int16_t) * HV_DIM);
