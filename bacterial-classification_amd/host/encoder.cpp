@@ -75,23 +75,30 @@ void HVCache::prefetch(const std::vector<std::string>& seqs) {
     for (size_t i = 0; i < todo.size(); ++i) prefetched.emplace(std::string(todo[i]), std::move(rows[i]));
 }
 
-void HVCache::insert(const std::string& key, HV_16 hv) {
+void HVCache::insert(const std::string& key, HV_16 hv, uint64_t vid) {
     if (map.size() >= capacity) {
         map.erase(lru.back());
         lru.pop_back();
     }
     lru.push_front(key);
-    map.emplace(key, Entry{std::move(hv), lru.begin()});
+    map.emplace(key, Entry{std::move(hv), lru.begin(), vid});
 }
 
-const HV_16& HVCache::get(const std::string& seq) {
+const HV_16& HVCache::get_tagged(const std::string& seq, uint64_t& vid) {
     auto it = map.find(seq);
     if (it != map.end()) {
         lru.splice(lru.begin(), lru, it->second.lru_it);
+        vid = it->second.vid;
         return it->second.hv;
     }
-    insert(seq, compute(seq));
+    insert(seq, compute(seq), 0);
+    vid = 0;
     return map.find(seq)->second.hv;
+}
+
+const HV_16& HVCache::get(const std::string& seq) {
+    uint64_t vid;
+    return get_tagged(seq, vid);
 }
 
 // Windows w = 0, 1, ... start at SIM_STEP * w.  Window 0 is get() of its
@@ -143,55 +150,74 @@ void HVCache::precompute(const std::string& seq) {
         const HV_16& sub = d[2 * step + 1];
         for (size_t j = 0; j < HV_DIM; ++j)
             window[j] = (int16_t)(window[j] + add[j] - sub[j] + z[step]);
-        insert(key, window);
+        insert(key, window, ++next_vid);
     }
 }
 
 SIM HVCache::similarity(const std::string& l, const std::string& r) {
     SIM sim{0, -1.0f};
     if (SEQUENCE_LENGTH >= r.size()) {
-        const HV_16& q = get(l);
+        // cosine(get(l), get(r)) in the reference: GCC evaluates the
+        // arguments right to left, so get(r) touches the cache first
         const HV_16 ref = get(r);
+        const HV_16& q = get(l);
         sim.score = cosine(q, ref);
         return sim;
     }
-    // gather the windows through get() in the reference's order
-    std::vector<HV_16> wins;
-    std::vector<size_t> pos;
+    // the windows through get(), each followed by get(l), the reference's
+    // order of cache touches (src/encoder.cpp:205-207).  get(l) returns the
+    // same value at every window unless an eviction in between replaced a
+    // sliding-window value by the exact one; consecutive windows that saw the
+    // same value of l form one segment.
+    std::vector<HV_16> wins, qs;
+    std::vector<size_t> pos, seg_end;  // segment i: windows [seg_end[i-1], seg_end[i])
+    uint64_t last_vid = ~uint64_t(0);
     for (size_t x = 0; x + SEQUENCE_LENGTH < r.size(); x += SIM_STEP) {
         wins.push_back(get(r.substr(x, SEQUENCE_LENGTH)));
         pos.push_back(x);
+        uint64_t vid = 0;
+        const HV_16& q = get_tagged(l, vid);
+        if (qs.empty() || vid != last_vid) {
+            if (!qs.empty()) seg_end.push_back(wins.size() - 1);
+            qs.push_back(q);
+            last_vid = vid;
+        }
     }
-    const HV_16 q = get(l);
-    std::vector<const HV_16*> wp;
-    for (const auto& w : wins) wp.push_back(&w);
+    seg_end.push_back(wins.size());
     auto& s = hvk_host::Session::instance();
-    const hvk_host::DeviceBundles qd = s.upload_counts({&q});
-    const hvk_host::DeviceBundles wd = s.upload_counts(wp);
-    // The reference keeps the first window whose float score is the maximum.
-    // Candidates come ordered by (c desc, index asc); distinct c may round to
-    // the same float, so take the earliest window among those whose float
-    // equals the best one.
-    int kk = (int)std::min<size_t>(wins.size(), 29);
-    hvk_host::Session::TopK t = s.search(qd, wd, kk);
-    const float best = two_c_minus_one(t.cos[0]);
-    auto earliest = [&](int n, int& j) {
-        size_t first = (size_t)t.idx[0];
-        for (j = 1; j < n && two_c_minus_one(t.cos[j]) == best; ++j)
-            first = std::min(first, (size_t)t.idx[j]);
-        return first;
-    };
-    int j = 0;
-    size_t first = earliest(kk, j);
-    if (j == kk && kk < (int)wins.size()) {
-        // the float tie runs past the candidate list: rank every window
-        kk = (int)wins.size();
-        t = s.search(qd, wd, kk);
-        first = earliest(kk, j);
-    }
-    if (best > sim.score) {
-        sim.score = best;
-        sim.pos = pos[first];
+    size_t w0 = 0;
+    for (size_t g = 0; g < qs.size(); ++g) {
+        const size_t w1 = seg_end[g], n = w1 - w0;
+        std::vector<const HV_16*> wp;
+        for (size_t w = w0; w < w1; ++w) wp.push_back(&wins[w]);
+        const hvk_host::DeviceBundles qd = s.upload_counts({&qs[g]});
+        const hvk_host::DeviceBundles wd = s.upload_counts(wp);
+        // The reference keeps the first window whose float score is the
+        // maximum.  Candidates come ordered by (c desc, index asc); distinct c
+        // may round to the same float, so take the earliest window among
+        // those whose float equals the best one.
+        int kk = (int)std::min<size_t>(n, 29);
+        hvk_host::Session::TopK t = s.search(qd, wd, kk);
+        const float best = two_c_minus_one(t.cos[0]);
+        auto earliest = [&](int m, int& j) {
+            size_t first = (size_t)t.idx[0];
+            for (j = 1; j < m && two_c_minus_one(t.cos[j]) == best; ++j)
+                first = std::min(first, (size_t)t.idx[j]);
+            return first;
+        };
+        int j = 0;
+        size_t first = earliest(kk, j);
+        if (j == kk && kk < (int)n) {
+            // the float tie runs past the candidate list: rank every window
+            kk = (int)n;
+            t = s.search(qd, wd, kk);
+            first = earliest(kk, j);
+        }
+        if (best > sim.score) {  // segments in window order: the earliest maximum
+            sim.score = best;
+            sim.pos = pos[w0 + first];
+        }
+        w0 = w1;
     }
     return sim;
 }

@@ -44,12 +44,19 @@ public:
     // not touched, so its state stays the reference's.
     void prefetch(const std::vector<std::string>& seqs);
 
+    // Extension: get() that also names the value it returns: vid = 0 when
+    // the entry holds compute(seq) (inserted by a get() miss), else a number
+    // unique to the one precompute() insertion whose sliding-window value it
+    // holds.  Equal (seq, vid) means equal values.
+    const HV_16& get_tagged(const std::string& seq, uint64_t& vid);
+
     std::unordered_map<std::string, HV> kmers;
 
 private:
     struct Entry {
         HV_16 hv;
         std::list<std::string>::iterator lru_it;
+        uint64_t vid;
     };
 
     size_t capacity;
@@ -57,7 +64,8 @@ private:
     std::unordered_map<std::string, Entry> map;
     size_t channel;
     std::unordered_map<std::string, HV_16> prefetched;
+    uint64_t next_vid = 0;
 
     HV_16 compute(const std::string& seq);
-    void insert(const std::string& key, HV_16 hv);
+    void insert(const std::string& key, HV_16 hv, uint64_t vid);
 };

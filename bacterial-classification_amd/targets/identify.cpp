@@ -3,17 +3,18 @@
 // read against a category by its best SEQUENCE_LENGTH window over the
 // category's reference sequences.  Same usage, logs and output format.
 //
-// The reference scores each (read, window) pair with a CPU cosine; here each
-// category is one GPU top-1 search of its candidate reads against all of its
-// windows (the max over refs of the max over windows is the top-1 window).
-// Window bundles come from HVCache::precompute, called per reference in the
-// reference's order, so cache hits and sliding-window values match.  The
+// The reference scores each (read, window) pair with a CPU cosine; here the
+// cache is touched in the reference's order (so the same entries are evicted
+// and every window holds the reference's value), and each category's cosines
+// are one GPU top-1 search of its candidate reads against its windows (the
+// max over refs of the max over windows is the top-1 window).  The
 // containers are the reference's where their iteration order reaches the
 // output (reads, categories, beams).
 #include <algorithm>
 #include <filesystem>
 #include <fstream>
 #include <iostream>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
@@ -48,18 +49,104 @@ struct Beam {
 // float(2c - 1) from an FP64 cosine, as cosine() forms it
 float score_of(double c) { return (float)(2.0 * (c - 0.5)); }
 
-// All window bundles similarity(read, ref) scans for every ref of a category,
-// gathered through the cache in the reference's order.
-void category_windows(HVCache& cache, const std::vector<std::string>& refs,
-                      std::vector<HV_16>& out) {
-    for (const auto& ref : refs) {
-        cache.precompute(ref);
-        if (SEQUENCE_LENGTH >= ref.size()) {
-            out.push_back(cache.get(ref));
-            continue;
+// The scores of one category's candidates: the reference's loop
+// (targets/identify.cpp:147-158) -- for each reference sequence,
+// precompute(ref), then similarity(read, ref) for each candidate in order --
+// replayed on the cache with the same touches (get(window) then get(read) per
+// window; get(ref) then get(read) for a ref of at most SEQUENCE_LENGTH), and
+// the cosines evaluated on the GPU.  The windows the first candidate sees
+// are the rows W; a candidate that saw the same value (get_tagged's vid) at
+// every window and one value of its read is scored by a top-1 search of its
+// read against W.  A candidate that did not (an eviction in between) keeps
+// its own (read value, window value) pairs, scored group by group.
+void score_category(HVCache& cache, hvk_host::Session& gpu, const std::vector<std::string>& refs,
+                    const std::vector<Beam*>& cands) {
+    const size_t nc = cands.size();
+    std::vector<HV_16> W, X, Q;  // window rows, extra window rows, read rows
+    std::vector<uint64_t> wvid;
+    std::map<std::pair<std::string, uint64_t>, int> q_row;
+    std::vector<int> qrow(nc, -1);
+    std::vector<uint64_t> qvid(nc, 0);
+    std::vector<char> irregular(nc, 0);
+    std::vector<std::vector<std::pair<int, int>>> pairs(nc);  // (Q row, W row or ~X row)
+    std::vector<size_t> wbase;
+    auto read_row = [&](const std::string& seq, uint64_t vid, const HV_16& v) {
+        auto it = q_row.find({seq, vid});
+        if (it != q_row.end()) return it->second;
+        Q.push_back(v);
+        q_row.emplace(std::make_pair(seq, vid), (int)Q.size() - 1);
+        return (int)Q.size() - 1;
+    };
+    // candidate c becomes irregular before window p of ref j: its pairs so far
+    auto materialize = [&](size_t c, size_t j, size_t p) {
+        irregular[c] = 1;
+        for (size_t jj = 0; jj <= j; ++jj) {
+            const size_t end = jj < j ? (jj + 1 < wbase.size() ? wbase[jj + 1] : W.size()) : wbase[j] + p;
+            for (size_t w = wbase[jj]; w < end; ++w) pairs[c].push_back({qrow[c], (int)w});
         }
-        for (size_t x = 0; x + SEQUENCE_LENGTH < ref.size(); x += SIM_STEP)
-            out.push_back(cache.get(ref.substr(x, SEQUENCE_LENGTH)));
+    };
+    for (size_t j = 0; j < refs.size(); ++j) {
+        const std::string& ref = refs[j];
+        cache.precompute(ref);
+        wbase.push_back(W.size());
+        const bool whole = SEQUENCE_LENGTH >= ref.size();
+        for (size_t c = 0; c < nc; ++c) {
+            const std::string& seq = cands[c]->sequence;
+            size_t p = 0;
+            for (size_t x = 0; whole ? x == 0 : x + SEQUENCE_LENGTH < ref.size();
+                 x += SIM_STEP, ++p) {
+                uint64_t wv = 0;
+                const HV_16& w = cache.get_tagged(whole ? ref : ref.substr(x, SEQUENCE_LENGTH), wv);
+                int wrow;
+                if (c == 0) {
+                    W.push_back(w);
+                    wvid.push_back(wv);
+                    wrow = (int)W.size() - 1;
+                } else if (!irregular[c] && wv == wvid[wbase[j] + p]) {
+                    wrow = (int)(wbase[j] + p);
+                } else {
+                    if (!irregular[c]) materialize(c, j, p);
+                    X.push_back(w);
+                    wrow = ~(int)(X.size() - 1);
+                }
+                uint64_t qv = 0;
+                const HV_16& q = cache.get_tagged(seq, qv);
+                if (qrow[c] < 0) {
+                    qrow[c] = read_row(seq, qv, q);
+                    qvid[c] = qv;
+                } else if (!irregular[c] && qv != qvid[c]) {
+                    materialize(c, j, p);
+                }
+                if (irregular[c]) pairs[c].push_back({read_row(seq, qv, q), wrow});
+            }
+        }
+    }
+    if (W.empty()) return;
+    auto score_of_c = [](double c) { return (float)(2.0 * (c - 0.5)); };
+    // regular candidates: one top-1 search of their reads against W
+    std::vector<int> regular_rows;
+    std::map<int, int> slot;  // Q row -> query slot
+    for (size_t c = 0; c < nc; ++c)
+        if (!irregular[c] && qrow[c] >= 0 && slot.emplace(qrow[c], (int)regular_rows.size()).second)
+            regular_rows.push_back(qrow[c]);
+    if (!regular_rows.empty()) {
+        std::vector<const HV_16*> qp, wp;
+        for (int r : regular_rows) qp.push_back(&Q[r]);
+        for (auto& w : W) wp.push_back(&w);
+        const auto top = gpu.search(gpu.upload_counts(qp), gpu.upload_counts(wp), 1);
+        for (size_t c = 0; c < nc; ++c)
+            if (!irregular[c] && qrow[c] >= 0)
+                cands[c]->score = std::max(cands[c]->score, score_of_c(top.cos[slot[qrow[c]]]));
+    }
+    // irregular candidates: per read value, a top-1 search over its windows
+    for (size_t c = 0; c < nc; ++c) {
+        if (!irregular[c]) continue;
+        std::map<int, std::vector<const HV_16*>> by_q;
+        for (auto [qr, wr] : pairs[c]) by_q[qr].push_back(wr >= 0 ? &W[wr] : &X[~wr]);
+        for (auto& [qr, wp] : by_q) {
+            const auto top = gpu.search(gpu.upload_counts({&Q[qr]}), gpu.upload_counts(wp), 1);
+            cands[c]->score = std::max(cands[c]->score, score_of_c(top.cos[0]));
+        }
     }
 }
 
@@ -128,32 +215,11 @@ int main(int argc, char* argv[]) {
             cur = std::move(next);
         }
 
-        // scoring: one GPU top-1 search per category
+        // scoring: the reference's cache touches, cosines on the GPU
         for (auto& [child, cands] : by_category) {
             const auto& refs = tree.get_sequences(child);
             std::cout << " Scoring category " << child << " (" << cands.size() << " candidates)\n";
-            std::vector<HV_16> windows;
-            category_windows(cache, refs, windows);
-            if (windows.empty() || cands.empty()) continue;
-            std::vector<const HV_16*> qrows;
-            std::unordered_map<std::string, int> row_of;
-            for (Beam* b : cands) {
-                if (row_of.emplace(b->sequence, (int)qrows.size()).second)
-                    qrows.push_back(&cache.get(b->sequence));
-            }
-            std::vector<HV_16> qcopy;
-            qcopy.reserve(qrows.size());
-            for (const HV_16* p : qrows) qcopy.push_back(*p);
-            std::vector<const HV_16*> qp, wp;
-            for (auto& q : qcopy) qp.push_back(&q);
-            for (auto& w : windows) wp.push_back(&w);
-            const auto qd = gpu.upload_counts(qp);
-            const auto wd = gpu.upload_counts(wp);
-            const auto top = gpu.search(qd, wd, 1);
-            for (Beam* b : cands) {
-                const float s = score_of(top.cos[row_of[b->sequence]]);
-                b->score = std::max(b->score, s);
-            }
+            score_category(cache, gpu, refs, cands);
         }
 
         for (auto& [target, bs] : beams)

@@ -8,6 +8,7 @@ mutated from its parent's (600 nt), so a read's path is recoverable; reads
 (250 nt) are sampled from species sequences with 1% substitutions, a few
 with N runs.
 """
+import gzip
 import os
 
 import numpy as np
@@ -62,3 +63,52 @@ def write(root, seed=7, n_reads=20, seq_len=600):
                 r = r[:p] + "N" * 12 + r[p + 12:]
             f.write(f">read{i}|{name}\n{r}\n")
     return root
+
+
+# ---- the cache-eviction taxonomy (tests/golden/make_evict_fixtures.py) ----
+
+EVICT_WIDTHS = (1, 4, 8, 16, 64, 128)  # categories per rank, phylum to species
+
+
+def evict_lineage(s):
+    """Category names from phylum to species of species index s (0..127)."""
+    return [f"{r}__E{s * w // 128}" for r, w in zip(RANKS, EVICT_WIDTHS)]
+
+
+def evict_tree(root, records):
+    """Pre-propagation layout: taxonomy_index.tsv (with the header line the
+    reference skips) and sequences_by_category/<species>.fasta."""
+    seqdir = os.path.join(root, "sequences_by_category")
+    os.makedirs(seqdir, exist_ok=True)
+    per = {}
+    for i, s in enumerate(records):
+        per.setdefault(i % 128, []).append(s)
+    lines = ["seq_id\tleaf\ttaxonomy"]
+    for sp in range(128):
+        lin = evict_lineage(sp)
+        lines.append(f"id{sp}\t{lin[-1]}\tk__Bacteria; " + "; ".join(lin))
+        with open(os.path.join(seqdir, lin[-1] + ".fasta"), "w") as f:
+            for j, s in enumerate(per.get(sp, [])):
+                f.write(f">{lin[-1]}_{j}\n{s}\n")
+    with open(os.path.join(root, "taxonomy_index.tsv"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    return root
+
+
+def evict_records(gold):
+    """The records (and short sequences) stored in gold/records.fasta.gz."""
+    with gzip.open(os.path.join(gold, "records.fasta.gz"), "rt") as f:
+        return _fasta_lines(f.read().splitlines())
+
+
+def _fasta_lines(lines):
+    out, cur = [], []
+    for line in lines:
+        if line.startswith(">"):
+            if cur or out:
+                out.append("".join(cur))
+            cur = []
+        elif line:
+            cur.append(line)
+    out.append("".join(cur))
+    return out
