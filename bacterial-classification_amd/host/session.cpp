@@ -182,19 +182,63 @@ std::vector<HV_16> Session::download_counts(const DeviceBundles& b) const {
 }
 
 Session::TopK Session::search(const DeviceBundles& q, const DeviceBundles& r, int k) const {
+    if (q.n == 0 || r.n == 0 || k <= 0) return search(hvk_bundles{}, hvk_bundles{}, k);
+    return search(q.view(q.needs_two_limbs()), r.view(r.needs_two_limbs()), k);
+}
+
+Session::TopK Session::search(const hvk_bundles& q, const hvk_bundles& r, int k) const {
     TopK t;
-    t.idx.assign((size_t)q.n * k, -1);
-    t.cos.assign((size_t)q.n * k, 0.0);
+    t.idx.assign((size_t)q.n * (k > 0 ? k : 0), -1);
+    t.cos.assign(t.idx.size(), 0.0);
     if (q.n == 0 || r.n == 0 || k <= 0) return t;
     DeviceBuffer idx(sizeof(int32_t) * q.n * k), score(sizeof(float) * q.n * k),
         cosv(sizeof(double) * q.n * k);
-    const hvk_bundles qv = q.view(q.needs_two_limbs()), rv = r.view(r.needs_two_limbs());
-    check(hvk_search_mfma_bounded(ctx_, &qv, &rv, ld_, k, idx.as<int32_t>(), score.as<float>(),
+    check(hvk_search_mfma_bounded(ctx_, &q, &r, ld_, k, idx.as<int32_t>(), score.as<float>(),
                                   cosv.as<double>(), nullptr, nullptr),
           "hvk_search_mfma_bounded");
     idx.download(t.idx.data(), sizeof(int32_t) * t.idx.size());
     cosv.download(t.cos.data(), sizeof(double) * t.cos.size());
     return t;
+}
+
+DeviceBundles Session::alloc(int n) const {
+    DeviceBundles b = alloc_bundles(n, ld_);
+    b.n = 0;
+    return b;
+}
+
+hvk_bundles Session::rows(const DeviceBundles& b, int i0, int i1, bool two_limbs) {
+    hvk_bundles v = b.view(two_limbs);
+    const int64_t o = (int64_t)i0 * b.ld;
+    v.clamp8 += o;
+    v.lo += o;
+    if (v.hi) v.hi += o;
+    v.sqnorm += i0;
+    v.maxabs += i0;
+    v.resid1 += i0;
+    v.n = i1 - i0;
+    return v;
+}
+
+void Session::append_row(DeviceBundles& dst, const DeviceBundles& src, int i) const {
+    const int j = dst.n;
+    if ((size_t)(j + 1) * sizeof(int64_t) > dst.sqnorm.bytes())
+        throw std::runtime_error("append_row: no room");
+    const size_t ld = (size_t)ld_;
+    auto cp = [&](const DeviceBuffer& s, DeviceBuffer& d, size_t row_bytes) {
+        hip_check(hipMemcpy((char*)d.get() + j * row_bytes, (const char*)s.get() + i * row_bytes,
+                            row_bytes, hipMemcpyDeviceToDevice),
+                  "hipMemcpy D2D");
+    };
+    cp(src.counts, dst.counts, ld * sizeof(int16_t));
+    cp(src.lo, dst.lo, ld);
+    cp(src.hi, dst.hi, ld);
+    cp(src.clamp8, dst.clamp8, ld);
+    cp(src.sqnorm, dst.sqnorm, sizeof(int64_t));
+    cp(src.maxabs, dst.maxabs, sizeof(int32_t));
+    cp(src.nkmers, dst.nkmers, sizeof(int32_t));
+    cp(src.resid1, dst.resid1, sizeof(int32_t));
+    dst.n = j + 1;
 }
 
 }  // namespace hvk_host

@@ -79,6 +79,15 @@ public:
     };
     // exact top-k (cosine desc, index asc) of q against r
     TopK search(const DeviceBundles& q, const DeviceBundles& r, int k) const;
+    // the same on operand views (rows of resident bundles)
+    TopK search(const hvk_bundles& q, const hvk_bundles& r, int k) const;
+
+    // device bundles with room for n rows (b.n = 0 rows in use)
+    DeviceBundles alloc(int n) const;
+    // rows [i0, i1) of b as a search operand
+    static hvk_bundles rows(const DeviceBundles& b, int i0, int i1, bool two_limbs);
+    // appends row i of src to dst (a device-to-device copy; dst has room)
+    void append_row(DeviceBundles& dst, const DeviceBundles& src, int i) const;
 
 private:
     Session();

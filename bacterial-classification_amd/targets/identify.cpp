@@ -46,9 +46,6 @@ struct Beam {
     }
 };
 
-// float(2c - 1) from an FP64 cosine, as cosine() forms it
-float score_of(double c) { return (float)(2.0 * (c - 0.5)); }
-
 // The scores of one category's candidates: the reference's loop
 // (targets/identify.cpp:147-158) -- for each reference sequence,
 // precompute(ref), then similarity(read, ref) for each candidate in order --

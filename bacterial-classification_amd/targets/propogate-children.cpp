@@ -4,14 +4,19 @@
 // (and those already added to it) is below 0.75 is appended to the parent's
 // FASTA.  Same usage, progress output and histogram.
 //
-// The descendants' scores against the parent's sequences at the start of the
-// parent are one GPU top-1 search; the sequentially growing set of sequences
-// added to the parent is scored with one more top-1 search per descendant.
+// The descendants' bundles are encoded once and stay on the device; so does
+// the set of sequences added to the parent, which grows by device row copies.
+// Descendants are scored in blocks of kBlock: one top-1 search of the block
+// against the parent's sequences and one against the sequences added before
+// the block, plus the block against itself (every pair, exact), so the
+// sequential acceptance inside the block runs on the host with no further
+// device work.
 #include <algorithm>
 #include <cmath>
 #include <iostream>
 #include <stdexcept>
 #include <string>
+#include <string_view>
 #include <vector>
 
 #include "../host/HV.h"
@@ -37,20 +42,21 @@ void collect(Hierarchy& tree, const std::string& node, std::vector<std::string>&
 
 float score_of(double c) { return (float)(2.0 * (c - 0.5)); }
 
-// best 2c - 1 of q against rows (top-1 on the GPU), -inf for no rows
-float best_against(hvk_host::Session& gpu, const std::vector<const HV_16*>& q,
-                   const std::vector<const HV_16*>& rows, std::vector<float>& out) {
-    out.assign(q.size(), -INFINITY);
-    if (rows.empty() || q.empty()) return -INFINITY;
-    const auto t = gpu.search(gpu.upload_counts(q), gpu.upload_counts(rows), 1);
-    for (size_t i = 0; i < q.size(); ++i) out[i] = score_of(t.cos[i]);
-    return out[0];
+constexpr int kBlock = 256;
+
+// best 2c - 1 of each row of q against r (top-1 on the GPU), -inf for no rows
+std::vector<float> best_against(hvk_host::Session& gpu, const hvk_bundles& q,
+                                const hvk_bundles& r) {
+    std::vector<float> out((size_t)q.n, -INFINITY);
+    if (q.n == 0 || r.n == 0) return out;
+    const auto t = gpu.search(q, r, 1);
+    for (int i = 0; i < q.n; ++i) out[i] = score_of(t.cos[i]);
+    return out;
 }
 
 }  // namespace
 
 int main(int argc, char* argv[]) {
-    HVCache cache;
     std::cout << "\nCache size is " << std::to_string(CACHE_SIZE) << " entries ("
               << std::to_string(CACHE_BYTES) << "b)\n";
     std::vector<int> counts(101, 0);
@@ -61,39 +67,52 @@ int main(int argc, char* argv[]) {
 
     for (size_t x = 1; x < kOrder.size(); ++x) {
         for (const auto& parent : tree.get_order(kOrder[x])) {
-            const auto& parent_seqs = tree.get_sequences(parent);
+            const std::vector<std::string> parent_seqs = tree.get_sequences(parent);
             std::vector<std::string> desc;
             collect(tree, parent, desc);
-            cache.prefetch(desc);
-            // descendants' bundles and their best score against the parent set
-            std::vector<HV_16> dh;
-            dh.reserve(desc.size());
-            for (const auto& d : desc) dh.push_back(cache.get(d));
-            std::vector<HV_16> ph;
-            for (const auto& p : parent_seqs) ph.push_back(cache.get(p));
-            std::vector<const HV_16*> dp, pp;
-            for (auto& h : dh) dp.push_back(&h);
-            for (auto& h : ph) pp.push_back(&h);
-            std::vector<float> vs_parent;
-            best_against(gpu, dp, pp, vs_parent);
-
-            std::vector<HV_16> added;
-            for (size_t i = 0; i < desc.size(); ++i) {
-                std::cout << "\rAnalyzing " << parent << " " << std::to_string(i) << "/"
-                          << std::to_string(desc.size()) << "        " << std::flush;
-                float best = std::max(0.0f, vs_parent[i]);
-                if (!added.empty()) {
-                    std::vector<const HV_16*> ap;
-                    for (auto& h : added) ap.push_back(&h);
-                    std::vector<float> s;
-                    best = std::max(best, best_against(gpu, {&dh[i]}, ap, s));
+            const int n = (int)desc.size();
+            // bundles on the device: descendants, the parent's sequences, and
+            // room for every descendant in the added set
+            std::vector<std::string_view> dv(desc.begin(), desc.end()),
+                pv(parent_seqs.begin(), parent_seqs.end());
+            hvk_host::DeviceBundles dh = gpu.encode(gpu.pack(dv));
+            hvk_host::DeviceBundles ph = gpu.encode(gpu.pack(pv));
+            hvk_host::DeviceBundles added = gpu.alloc(std::max(n, 1));
+            // hi limbs wherever any bundle needs them (exact either way)
+            const bool two = dh.needs_two_limbs() || ph.needs_two_limbs();
+            const std::vector<float> vs_parent =
+                best_against(gpu, hvk_host::Session::rows(dh, 0, n, two),
+                             hvk_host::Session::rows(ph, 0, ph.n, two));
+            std::vector<int> block_added;  // block offsets of this block's acceptances
+            for (int i0 = 0; i0 < n; i0 += kBlock) {
+                const int i1 = std::min(n, i0 + kBlock), m = i1 - i0;
+                const hvk_bundles blk = hvk_host::Session::rows(dh, i0, i1, two);
+                const std::vector<float> vs_added =
+                    best_against(gpu, blk, hvk_host::Session::rows(added, 0, added.n, two));
+                // every pair of the block: s[a][b] = score of descendant i0+a vs i0+b
+                const auto t = gpu.search(blk, blk, m);
+                std::vector<float> pair((size_t)m * m, -INFINITY);
+                for (int a2 = 0; a2 < m; ++a2)
+                    for (int j = 0; j < m; ++j) {
+                        const int b2 = t.idx[(size_t)a2 * m + j];
+                        if (b2 >= 0) pair[(size_t)a2 * m + b2] = score_of(t.cos[(size_t)a2 * m + j]);
+                    }
+                block_added.clear();
+                for (int a2 = 0; a2 < m; ++a2) {
+                    const int i = i0 + a2;
+                    std::cout << "\rAnalyzing " << parent << " " << std::to_string(i) << "/"
+                              << std::to_string(n) << "        " << std::flush;
+                    float best = std::max(0.0f, vs_parent[i]);
+                    best = std::max(best, vs_added[a2]);
+                    for (int b2 : block_added) best = std::max(best, pair[(size_t)a2 * m + b2]);
+                    counts[(int)std::floor(best)]++;
+                    if (best < kThreshold) {
+                        tree.append_sequence(parent, desc[i]);
+                        block_added.push_back(a2);
+                        total += 1;
+                    }
                 }
-                counts[(int)std::floor(best)]++;
-                if (best < kThreshold) {
-                    tree.append_sequence(parent, desc[i]);
-                    added.push_back(dh[i]);
-                    total += 1;
-                }
+                for (int b2 : block_added) gpu.append_row(added, dh, i0 + b2);
             }
         }
     }

@@ -43,6 +43,69 @@ def pmc_traffic(kernel, reads, refs):
     return None
 
 
+VALU_PROFILE = os.path.join(ROOT, "profiles", "encode_config1_pmc.md")
+NOMINAL_CLOCK_GHZ = 2.4  # MI355X peak engine clock
+
+
+def valu_counters(kernel, reads):
+    """Per-launch SQ counters of `kernel` on `reads` reads from the committed
+    rocprofv3 --pmc summary, or None when it does not cover them."""
+    try:
+        text = open(VALU_PROFILE).read()
+    except OSError:
+        return None
+    for m in re.finditer(r"valu_per_launch: kernel=(\S+) reads=(\d+) (.*)", text):
+        if m.group(1) == kernel and int(m.group(2)) == reads:
+            return {k: float(v) for k, v in re.findall(r"(\w+)=([0-9.eE+]+)", m.group(3))}
+    return None
+
+
+def cpu_rows(w, n_take, db_host, dbn, n_refs, dim, kmer, topk, gpu_idx, cpu_seconds):
+    """The CPU baseline: the C restatement of the reference's bit-packed path
+    (ctz bundling of on-the-fly bound k-mer HVs, the int64 cosine loop, top-k)
+    built here with the reference's Release flags for this host's CPU, timed
+    on one thread and on every core this process may use (at most 16, the
+    box's CPU share per GPU), each on the first reads of the workload (sample
+    sized for about cpu_seconds / 2 each).  Top-k must equal the GPU's."""
+    import tempfile
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as orc
+    tmp = tempfile.mkdtemp(prefix="hvk_cpu_")
+    O, flags = orc.release_classifier(tmp)
+    if O is None:
+        O, flags = orc.oracle(), "prebuilt liboracle.so (-O3 -march=x86-64-v3): " + flags
+    nproc, avail, model = orc.host_cpu()
+    n_all = max(1, min(16, avail))
+
+    def run(n, threads):
+        seqs = w.reads[: w.read_offsets[n]]
+        offs = np.ascontiguousarray(w.read_offsets[: n + 1])
+        oi = np.zeros((n, topk), np.int32)
+        oc = np.zeros((n, topk), np.float64)
+        sec = O.orc_classify_reads(dim, kmer, seqs, offs.ctypes.data, n, db_host.ctypes.data,
+                                   dbn.ctypes.data, n_refs, topk, oi.ctypes.data, oc.ctypes.data,
+                                   threads)
+        return sec, oi
+
+    rows = {}
+    for name, threads in (("cpu_1", 1), ("cpu_all", n_all)):
+        t1, _ = run(min(threads, n_take), threads)  # one read per thread: calibrates the sample
+        n = int(max(min(threads, n_take),
+                    min(n_take, (cpu_seconds / 2.0) / t1 * threads // threads * threads)))
+        sec, oi = run(n, threads)
+        rows[name] = {"value": n / sec, "unit": "reads/s", "threads": threads, "reads": n,
+                      "seconds": sec, "topk_equal_gpu": bool(np.array_equal(oi, gpu_idx[:n]))}
+        print(f"[bench] {name}: {n / sec:.2f} reads/s ({n} reads, {threads} threads, {sec:.1f}s)",
+              file=sys.stderr, flush=True)
+    a = rows["cpu_all"]
+    return {"value": a["value"], "unit": "reads/s", "cores": n_all, "kind": "port",
+            "sample": f"first {a['reads']} reads x all {n_refs} refs, top-{topk} ({a['seconds']:.1f}s "
+                      f"wall, {n_all} threads); 1-thread row: first {rows['cpu_1']['reads']} reads",
+            "flags": flags, "nproc": nproc, "cores_available": avail, "cpu_model": model,
+            "cpu_1": rows["cpu_1"], "cpu_all": a,
+            "top5_equal_gpu": bool(a["topk_equal_gpu"] and rows["cpu_1"]["topk_equal_gpu"])}
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -50,14 +113,18 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--reads", type=int, default=1_000_000)
     ap.add_argument("--ancestors", type=int, default=2000)
-    ap.add_argument("--plain-reads", type=int, default=16384)
+    ap.add_argument("--plain-reads", type=int, default=262144,
+                    help="reads of the plain-form runs (median of --plain-runs)")
+    ap.add_argument("--plain-runs", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU baseline budget")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-two-limb", action="store_true")
-    ap.add_argument("--workload", choices=["classify", "precision", "descent", "batch"],
+    ap.add_argument("--workload", choices=["classify", "precision", "descent", "batch", "config0"],
                     default="classify",
-                    help="classify: the reads/s line (1M reads x 100k refs, top-5); precision: "
+                    help="classify: the reads/s line (1M reads x 100k refs, top-5); config0: D=8192, "
+                         "k=10, 1,000 refs, 500 reads of 250 nt, top-1 (the CPU-parity workload); "
+                         "precision: "
                          "per-result error vs FP64 at 500k refs, 2M reads of 250 nt, top-10; "
                          "descent: top-3 beam down a 7-rank taxonomy of 60,000 leaves, 1M reads; "
                          "batch: the batching form on 4M reads of 100-1,600 nt, 100k refs, top-5")
@@ -91,6 +158,9 @@ def main():
         return
     if a.workload == "batch":
         run_batch(a)
+        return
+    if a.workload == "config0":
+        run_config0(a)
         return
     if a.copy_calib_mb > 0:
         nbytes = a.copy_calib_mb << 20
@@ -198,60 +268,39 @@ def main():
     enc_bytes = float(np.sum((lens + 3) // 4)) + nq * (3.0 * ld) + nq * 20
     enc_gbs = enc_bytes / e_kernel / 1e9
 
-    # plain form on a subset
+    # plain form on a subset: median of plain_runs timed runs
     plain = None
     if not a.no_plain:
         npl = min(a.plain_reads, nq)
         sub = H.DeviceSeqs(w.reads[: w.read_offsets[npl]], w.read_offsets[: npl + 1])
         qp = H.encode(ctx, sub, counts=True, limbs=False)
         torch.cuda.synchronize()
-        H.search_plain(ctx, qp, db, TOPK)  # warm
+        resp = H.search_plain(ctx, qp, db, TOPK)  # warm
         torch.cuda.synchronize()
-        tp = time.time()
-        resp = H.search_plain(ctx, qp, db, TOPK)
-        torch.cuda.synchronize()
-        plain_s = time.time() - tp
-        _, plain_kernel_ms = ctx.last_timing()
+        runs, kms = [], []
+        for _ in range(a.plain_runs):
+            tp = time.time()
+            resp = H.search_plain(ctx, qp, db, TOPK, out=resp)
+            torch.cuda.synchronize()
+            runs.append(time.time() - tp)
+            kms.append(ctx.last_timing()[1])
+        plain_s = float(np.median(runs))
+        plain_kernel_ms = float(np.median(kms))
         same = bool(torch.equal(resp.idx, out.idx[:npl]))
-        plain = {"reads": npl, "reads_per_s": npl / plain_s,
+        plain = {"reads": npl, "runs": a.plain_runs, "reads_per_s": npl / plain_s,
+                 "reads_per_s_runs": [npl / t for t in runs],
                  "kernel_ms": plain_kernel_ms, "top5_equal_mfma": same,
                  "valu_int_tops": 2.0 * DIM * n_refs * npl / (plain_kernel_ms / 1000.0) / 1e12}
-        print(f"[bench] plain: {npl / plain_s:.0f} reads/s (top-5 equal: {same})",
-              file=sys.stderr, flush=True)
+        print(f"[bench] plain: {npl / plain_s:.0f} reads/s median of {a.plain_runs} "
+              f"(top-5 equal: {same})", file=sys.stderr, flush=True)
+        del qp, resp
 
     # CPU baseline: the oracle's restatement of the reference's bit-packed path
     cpu = None
     if not a.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle_py as orc
-        ncores = min(16, os.cpu_count() or 1)
-        db_host = db.counts.cpu().numpy()
-        if ld != DIM:
-            db_host = np.ascontiguousarray(db_host[:, :DIM])
-        dbn = db.sqnorm.cpu().numpy()
-        O = orc.oracle()
-
-        def cpu_run(n):
-            seqs = w.reads[: w.read_offsets[n]]
-            offs = np.ascontiguousarray(w.read_offsets[: n + 1])
-            oi = np.zeros((n, TOPK), np.int32)
-            oc = np.zeros((n, TOPK), np.float64)
-            sec = O.orc_classify_reads(DIM, KMER, seqs, offs.ctypes.data, n, db_host.ctypes.data,
-                                       dbn.ctypes.data, n_refs, TOPK, oi.ctypes.data,
-                                       oc.ctypes.data, ncores)
-            return sec, oi, oc
-
-        sec1, _, _ = cpu_run(ncores)  # one read per thread: calibrates the sample size
-        per_read = sec1 / ncores
-        n_cpu = int(max(ncores, min(nq, (a.cpu_seconds / per_read) // ncores * ncores)))
-        sec, oi, oc = cpu_run(n_cpu)
-        gpu_idx = out.idx[:n_cpu].cpu().numpy()
-        cpu = {"value": n_cpu / sec, "unit": "reads/s", "cores": ncores, "kind": "port",
-               "sample": f"first {n_cpu} reads x all {n_refs} refs, top-5 "
-                         f"({sec:.1f}s wall, {ncores} threads)",
-               "top5_equal_gpu": bool(np.array_equal(oi, gpu_idx))}
-        print(f"[bench] cpu: {n_cpu / sec:.1f} reads/s on {ncores} threads, top-5 equal "
-              f"{cpu['top5_equal_gpu']}", file=sys.stderr, flush=True)
+        db_host = np.ascontiguousarray(db.counts.cpu().numpy()[:, :DIM])
+        cpu = cpu_rows(w, nq, db_host, db.sqnorm.cpu().numpy(), n_refs, DIM, KMER, TOPK,
+                       out.idx.cpu().numpy(), a.cpu_seconds)
 
     # the bounded form keeps KP-entry lists: 8 up to top-5, 16 up to top-13, else 32
     bounded_kernel = f"mfma_kernel<1,1,{8 if TOPK <= 5 else (16 if TOPK <= 13 else 32)},true>"
@@ -274,8 +323,7 @@ def main():
                                        "(copy-calibrated), profiles/mfma_kernel_config1_pmc.md",
                      "kernel": bounded_kernel,
                      "kernel_ms": float(np.mean(srch_ms))},
-        "encode": {"kernel_ms": float(np.mean(enc_ms)), "achieved_gbs": enc_gbs,
-                   "frac_hbm": enc_gbs / HBM_PEAK_GBS},
+        "encode": encode_line(enc_ms, enc_gbs, nq, lens),
         "mfma_two_limb": two,
         "plain": plain,
         "mfma_vs_plain": (value / plain["reads_per_s"]) if plain else None,
@@ -284,6 +332,31 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def encode_line(enc_ms, enc_gbs, nq, lens):
+    """The encoder's measurements: HBM bytes (its algorithmic traffic) and,
+    from the committed SQ counter profile of the same launch, its VALU issue
+    against the chip's peak (one wave64 VALU instruction per CU per clock at
+    the nominal clock)."""
+    import torch
+    kernel = "encode_pair_kernel<12,11>"
+    t = float(np.mean(enc_ms)) / 1000.0
+    line = {"kernel": kernel, "kernel_ms": 1000.0 * t, "achieved_gbs": enc_gbs,
+            "frac_hbm": enc_gbs / HBM_PEAK_GBS, "roofline": None}
+    c = valu_counters(kernel, nq)
+    if c:
+        n_cu = torch.cuda.get_device_properties(0).multi_processor_count
+        peak = n_cu * NOMINAL_CLOCK_GHZ  # G wave-instructions per second
+        achieved = c["SQ_INSTS_VALU"] / t / 1e9
+        kmers = float(np.sum(np.maximum(lens - KMER + 1, 0)))
+        line["roofline"] = {
+            "bound": "valu", "achieved": achieved, "peak": peak, "unit": "G wave-instr/s",
+            "frac": achieved / peak,
+            "valu_insts_per_kmer_word": c["SQ_INSTS_VALU"] * 64 / (kmers * ((DIM + 31) // 32)),
+            "lds_insts_per_valu": c.get("SQ_INSTS_LDS", 0.0) / c["SQ_INSTS_VALU"],
+            "source": "rocprofv3 --pmc SQ_INSTS_VALU et al., profiles/encode_config1_pmc.md"}
+    return line
 
 
 def run_precision(a):
@@ -468,7 +541,8 @@ def run_descent(a):
     if not a.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle_py as orc
-        ncores = min(16, os.cpu_count() or 1)
+        nproc, avail, model = orc.host_cpu()
+        ncores = max(1, min(16, avail))
         proto = tree.proto.cpu().numpy()[:, :DIM]
         qc_all = reads.counts  # device
         n_cpu = ncores
@@ -486,6 +560,8 @@ def run_descent(a):
                     np.array_equal(ref[1].view(np.int32), out.score[:n_cpu].cpu().numpy().view(np.int32)))
         cpu = {"value": n_cpu / tc, "unit": "reads/s", "cores": ncores, "kind": "port",
                "sample": f"first {n_cpu} reads (encoded), six levels ({tc:.1f}s wall)",
+               "nproc": nproc, "cores_available": avail, "cpu_model": model,
+               "flags": "prebuilt liboracle.so (-O3 -march=x86-64-v3)",
                "paths_and_scores_equal_gpu": same}
         print(f"[descent] cpu: {n_cpu / tc:.1f} reads/s, equal {same}", file=sys.stderr, flush=True)
 
@@ -573,29 +649,9 @@ def run_batch(a):
 
     cpu = None
     if not a.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle_py as orc
-        ncores = min(16, os.cpu_count() or 1)
         db_host = np.ascontiguousarray(db.counts.cpu().numpy()[:, :DIM])
-        dbn = db.sqnorm.cpu().numpy()
-        O = orc.oracle()
-
-        def cpu_run(n):
-            oi = np.zeros((n, TOPK), np.int32)
-            oc = np.zeros((n, TOPK), np.float64)
-            offs = np.ascontiguousarray(w.read_offsets[: n + 1])
-            t = O.orc_classify_reads(DIM, KMER, w.reads[: w.read_offsets[n]], offs.ctypes.data, n,
-                                     db_host.ctypes.data, dbn.ctypes.data, n_refs, TOPK,
-                                     oi.ctypes.data, oc.ctypes.data, ncores)
-            return t, oi
-
-        t1, _ = cpu_run(ncores)
-        n_cpu = int(max(ncores, min(nq, (a.cpu_seconds / (t1 / ncores)) // ncores * ncores)))
-        tc, oi = cpu_run(n_cpu)
-        cpu = {"value": n_cpu / tc, "unit": "reads/s", "cores": ncores, "kind": "port",
-               "sample": f"first {n_cpu} reads x all {n_refs} refs, top-5 ({tc:.1f}s wall)",
-               "top5_equal_gpu": bool(np.array_equal(oi, out.idx[:n_cpu].cpu().numpy()))}
-        print(f"[batch] cpu: {n_cpu / tc:.1f} reads/s", file=sys.stderr, flush=True)
+        cpu = cpu_rows(w, nq, db_host, db.sqnorm.cpu().numpy(), n_refs, DIM, KMER, TOPK,
+                       out.idx.cpu().numpy(), a.cpu_seconds)
 
     line = {
         "metric": "reads/s classified, batching form (D=10240, 100k refs, 4M reads of "
@@ -612,6 +668,94 @@ def run_batch(a):
         "graphs": b.get_config()["graphs"],
         "direct_path": {"reads": nd, "reads_per_s": nd / direct_s, "top5_equal_batch": same},
         "n_rescored": int(out.n_rescored.item()),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def run_config0(a):
+    """The CPU-parity workload: D=8192, k=10; 20 random ancestors of 1,500 nt,
+    50 descendants each at 3% substitution and 1% indels (1,000 references),
+    500 reads of 250 nt sampled from the references with 1% substitutions;
+    top-1.  One step = encode the 500 reads + the bounded MFMA search with its
+    exact finalize; inputs resident in HBM.  Also timed: the plain and
+    two-limb GPU forms, and the CPU baseline on all 500 reads (its own
+    workload: the config names the reference's CPU method)."""
+    import torch
+    import hdc_amd as H
+    D0, K0, T0 = 8192, 10, 1
+    w = H.generate(H.make_spec())
+    n_refs, nq = len(w.ref_offsets) - 1, len(w.read_offsets) - 1
+    ctx = H.Context(D0, K0)
+    db = H.encode(ctx, H.DeviceSeqs(w.refs, w.ref_offsets), counts=True, limbs=True, hi=True,
+                  clamp=True)
+    seqs = H.DeviceSeqs(w.reads, w.read_offsets)
+    L = H.lib()
+    reads = H._alloc(nq, ctx.ld, counts=True, limbs=True, hi=True, clamp=True)
+    ro = H._out_struct(reads, ctx.ld)
+    out = H._topk_bufs(nq, T0)
+    torch.cuda.synchronize()
+    db_b = H.bundles(db)
+
+    def encode():
+        H.check(L.hvk_encode_ex(ctx.h, H._ptr(seqs.packed), H._ptr(seqs.invalid),
+                                H._ptr(seqs.offsets), nq, seqs.max_len, H.ctypes.byref(ro),
+                                H._stream()), "hvk_encode_ex")
+
+    encode()
+    torch.cuda.synchronize()
+    q_b = H.bundles(reads)
+
+    def step():
+        encode()
+        H.check(L.hvk_search_mfma_bounded(ctx.h, H.ctypes.byref(q_b), H.ctypes.byref(db_b), ctx.ld,
+                                          T0, H._ptr(out.idx), H._ptr(out.score), H._ptr(out.cos64),
+                                          H._ptr(out.n_rescored), H._stream()),
+                "hvk_search_mfma_bounded")
+
+    steps = max(a.steps, 200)
+    for _ in range(max(a.warmup, 5)):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.time()
+    for _ in range(steps):
+        step()
+    torch.cuda.synchronize()
+    sec = (time.time() - t0) / steps
+    forms = {}
+    for name, fn in (("plain", lambda: H.search_plain(ctx, reads, db, T0)),
+                     ("two_limb", lambda: H.search_mfma(ctx, reads, db, T0))):
+        r = fn()
+        torch.cuda.synchronize()
+        ts = time.time()
+        for _ in range(50):
+            encode()
+            r = fn()
+        torch.cuda.synchronize()
+        forms[name] = {"reads_per_s": nq * 50 / (time.time() - ts),
+                       "top1_equal_bounded": bool(torch.equal(r.idx, out.idx))}
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as orc
+    qc = reads.counts.cpu().numpy()[:, :D0]
+    rc = db.counts.cpu().numpy()[:, :D0]
+    ref_idx, ref_cos = orc.topk16(qc, rc, T0)
+    sc = out.score.cpu().numpy().astype(np.float64)
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_rows(w, nq, np.ascontiguousarray(rc), db.sqnorm.cpu().numpy(), n_refs, D0, K0,
+                       T0, out.idx.cpu().numpy(), a.cpu_seconds)
+    line = {
+        "metric": "reads/s classified (D=8192, k=10, 1,000 refs, 500 reads of 250 nt, top-1)",
+        "value": nq / sec, "unit": "reads/s", "n_gpus": 1, "steps": steps, "warmup": a.warmup,
+        "ms_per_step": 1000.0 * sec, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int8", "data": "synthetic (deterministic generator, seed 0)",
+        "config": {"workload": "config 0: D=8192, k=10, 20 ancestors x 50 refs of 1,500 nt "
+                               "(3% subst, 1% indel), 500 reads of 250 nt (1% error), top-1",
+                   "reads": nq, "refs": n_refs, "dim": D0, "k": K0, "topk": T0},
+        "top1_equal_oracle": bool(np.array_equal(out.idx.cpu().numpy(), ref_idx)),
+        "score_max_rel_err": float((np.abs(sc - ref_cos) / np.abs(ref_cos)).max()),
+        "truth_hit_rate": float((out.idx.cpu().numpy()[:, 0] == w.truth).mean()),
+        "forms": forms,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

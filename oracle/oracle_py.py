@@ -194,3 +194,49 @@ def tax_order(tsv_path, names):
     if m < 0:
         raise ValueError("candidate lists exceed capacity")
     return off, idx[:m].copy(), roots[: nr.value].copy()
+
+
+# ---- CPU baseline build -----------------------------------------------------
+RELEASE_FLAGS = ["-O3", "-march=native", "-ffast-math"]  # the reference's Release (src/CMakeLists.txt:28)
+
+
+def release_classifier(out_dir):
+    """orc_classify_reads (the C restatement of the reference's bit-packed CPU
+    path) compiled here with the reference's Release flags for this host's
+    CPU, into out_dir.  Returns (ctypes library, flags) or (None, error text)
+    when gcc is unavailable or fails."""
+    import subprocess
+    so = os.path.join(out_dir, "liboracle_release.so")
+    obj = os.path.join(out_dir, "hdc_oracle_release.o")
+    try:
+        subprocess.run(["gcc", *RELEASE_FLAGS, "-fPIC", "-c", os.path.join(HERE, "hdc_oracle.c"),
+                        "-o", obj], check=True, capture_output=True, text=True)
+        subprocess.run(["gcc", "-shared", "-o", so, obj, "-lpthread", "-lm"], check=True,
+                       capture_output=True, text=True)
+    except (OSError, subprocess.CalledProcessError) as e:
+        return None, str(getattr(e, "stderr", e))[-500:]
+    L = ctypes.CDLL(so)
+    vp, i32 = ctypes.c_void_p, ctypes.c_int
+    L.orc_classify_reads.restype = ctypes.c_double
+    L.orc_classify_reads.argtypes = [i32, i32, ctypes.c_char_p, vp, i32, vp, vp, i32, i32, vp, vp,
+                                     i32]
+    return L, " ".join(RELEASE_FLAGS)
+
+
+def host_cpu():
+    """(nproc, cores this process may use, CPU model name)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return os.cpu_count() or 1, avail, model
+
