@@ -30,7 +30,7 @@
 #include <cstring>
 #include <vector>
 
-#include "hvk_internal.h"
+#include "search_impl.h"
 
 namespace {
 
@@ -41,7 +41,7 @@ struct GraphKey {
     int64_t c0;
     int n;
     int k;
-    int form, splits;
+    int form, shape, splits;
     int width;          // counter width class of the chunk's longest read
     hvk_bundles db;
     const void* in;     // the batch's buffers (move when they grow)
@@ -62,6 +62,7 @@ struct Graph {
 struct hvk_batch {
     hvk_ctx* ctx = nullptr;
     int form = HVK_BATCH_FORM_BOUNDED;
+    int shape = 0;
     int splits = 0;
     int chunk = 0;
     hipStream_t s = nullptr;
@@ -170,6 +171,7 @@ int run_chunk(hvk_batch* b, const uint32_t* packed, const uint32_t* invalid, int
                               lens + c0))
         return e;
     ctx->tune_splits = b->splits;
+    ctx->tune_shape = b->shape;
     int e;
     if (b->form == HVK_BATCH_FORM_BOUNDED) {
         hvk_bundles q{cb.clamp8, cb.lo, cb.hi, cb.sqnorm, cb.maxabs, cb.resid1, n};
@@ -281,6 +283,7 @@ int classify(hvk_batch* b, const uint32_t* caller_packed, const uint32_t* caller
             key.n = m;
             key.k = k;
             key.form = b->form;
+            key.shape = b->shape;
             key.splits = b->splits;
             key.width = width_class(chunk_max, b->ctx->k);
             key.db.clamp8 = db.clamp8;  // field by field: the struct's padding stays zero
@@ -386,24 +389,26 @@ extern "C" int hvk_batch_destroy(hvk_batch* b) {
     return hvk_ctx_destroy(ctx);
 }
 
-extern "C" int hvk_batch_config(hvk_batch* b, int form, int splits, int chunk) {
-    if (!b || (form != HVK_BATCH_FORM_BOUNDED && form != HVK_BATCH_FORM_TWO_LIMB) || splits < 0 ||
-        splits > 64 || chunk < kMinChunk)
+extern "C" int hvk_batch_config(hvk_batch* b, int form, int shape, int splits, int chunk) {
+    if (!b || (form != HVK_BATCH_FORM_BOUNDED && form != HVK_BATCH_FORM_TWO_LIMB) || shape < 0 ||
+        shape >= kMfmaShapes || splits < 0 || splits > 64 || chunk < kMinChunk)
         return HVK_ERR_ARG;
-    if (form != b->form || splits != b->splits || chunk != b->chunk) {
+    if (form != b->form || shape != b->shape || splits != b->splits || chunk != b->chunk) {
         HVK_HIP(hipStreamSynchronize(b->s));
         drop_graphs(b);
     }
     b->form = form;
+    b->shape = shape;
     b->splits = splits;
     b->chunk = chunk;
     return HVK_OK;
 }
 
-extern "C" int hvk_batch_get_config(const hvk_batch* b, int* form, int* splits, int* chunk,
-                                    int* n_graphs) {
+extern "C" int hvk_batch_get_config(const hvk_batch* b, int* form, int* shape, int* splits,
+                                    int* chunk, int* n_graphs) {
     if (!b) return HVK_ERR_ARG;
     if (form) *form = b->form;
+    if (shape) *shape = b->shape;
     if (splits) *splits = b->splits;
     if (chunk) *chunk = b->chunk;
     if (n_graphs) *n_graphs = (int)b->graphs.size();
@@ -423,26 +428,23 @@ extern "C" int hvk_batch_classify(hvk_batch* b, const uint32_t* d_packed,
 extern "C" int hvk_batch_autotune(hvk_batch* b, const uint32_t* d_packed,
                                   const uint32_t* d_invalid, const int64_t* h_offsets, int n,
                                   const hvk_bundles* db, int k, int32_t* d_idx, float* d_score,
-                                  int sample, float* ms_per_mread, hvk_stream stream) {
+                                  int sample, float* ms_per_mread, float* table, int* table_rows,
+                                  hvk_stream stream) {
     if (int e = check_inputs(b, d_packed, d_invalid, h_offsets, n, db, k, d_idx, d_score)) return e;
     if (n == 0 || sample <= 0) return HVK_ERR_ARG;
+    if (table && !table_rows) return HVK_ERR_ARG;
     const int m = std::min(n, sample);
-    struct Cand { int form, splits, chunk; };
-    std::vector<Cand> cands;
-    for (int chunk : {m / 4, m / 2, m})
-        for (int s : {4, 8, 16})
-            if (chunk >= kMinChunk) cands.push_back({HVK_BATCH_FORM_BOUNDED, s, chunk});
-    cands.push_back({HVK_BATCH_FORM_TWO_LIMB, 4, std::max(kMinChunk, m / 2)});
     hipStream_t st = (hipStream_t)stream;
-    float best = -1.0f;
-    Cand pick = {b->form, b->splits, b->chunk};
-    for (const Cand& c : cands) {
-        if (int e = hvk_batch_config(b, c.form, c.splits, c.chunk)) return e;
-        // direct run + capture, then the faster of two timed replays
+    struct Cand { int form, shape, splits, chunk; };
+    const int cap = table ? *table_rows : 0;
+    int rows = 0;
+    // times one setting: a direct run and capture, then the faster of two replays
+    auto time_one = [&](const Cand& c, float& ms) -> int {
+        if (int e = hvk_batch_config(b, c.form, c.shape, c.splits, c.chunk)) return e;
         if (int e = classify(b, d_packed, d_invalid, h_offsets, m, *db, k, d_idx, d_score, nullptr,
                              nullptr, st))
             return e;
-        float ms = -1.0f;
+        ms = -1.0f;
         for (int rep = 0; rep < 2; ++rep) {
             HVK_HIP(hipEventRecord(b->t0, st));
             if (int e = classify(b, d_packed, d_invalid, h_offsets, m, *db, k, d_idx, d_score,
@@ -454,12 +456,46 @@ extern "C" int hvk_batch_autotune(hvk_batch* b, const uint32_t* d_packed,
             HVK_HIP(hipEventElapsedTime(&t, b->t0, b->t1));
             if (ms < 0.0f || t < ms) ms = t;
         }
+        if (rows < cap) {
+            float* row = table + 5 * (size_t)rows;
+            row[0] = (float)c.form;
+            row[1] = (float)c.shape;
+            row[2] = (float)c.splits;
+            row[3] = (float)c.chunk;
+            row[4] = ms * 1e6f / (float)m;
+        }
+        ++rows;
+        return HVK_OK;
+    };
+    float best = -1.0f;
+    Cand pick = {b->form, b->shape, b->splits, b->chunk};
+    auto consider = [&](const Cand& c) -> int {
+        float ms = 0.0f;
+        if (int e = time_one(c, ms)) return e;
         if (best < 0.0f || ms < best) {
             best = ms;
             pick = c;
         }
-    }
-    if (int e = hvk_batch_config(b, pick.form, pick.splits, pick.chunk)) return e;
+        return HVK_OK;
+    };
+    // Coordinate search: the tile shape (the bounded form's shapes apply to
+    // top-5, 8-entry lists; other k keep the default), then the reference
+    // slices, then the chunk size, each with the others at their best so
+    // far; and the two-limb form once.
+    const int half = std::max(kMinChunk, m / 2);
+    const int n_shapes = pick_kp(k) == 8 ? kMfmaShapes : 1;
+    for (int sh = 0; sh < n_shapes; ++sh)
+        if (int e = consider({HVK_BATCH_FORM_BOUNDED, sh, 0, half})) return e;
+    const int best_shape = pick.shape;
+    for (int s : {4, 8, 16})
+        if (int e = consider({HVK_BATCH_FORM_BOUNDED, best_shape, s, half})) return e;
+    const int best_splits = pick.splits;
+    for (int chunk : {m / 4, m})
+        if (chunk >= kMinChunk && chunk != half)
+            if (int e = consider({HVK_BATCH_FORM_BOUNDED, best_shape, best_splits, chunk})) return e;
+    if (int e = consider({HVK_BATCH_FORM_TWO_LIMB, 0, 4, half})) return e;
+    if (int e = hvk_batch_config(b, pick.form, pick.shape, pick.splits, pick.chunk)) return e;
     if (ms_per_mread) *ms_per_mread = best * 1e6f / (float)m;
+    if (table_rows) *table_rows = std::min(rows, cap);
     return HVK_OK;
 }

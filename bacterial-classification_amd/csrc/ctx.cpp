@@ -157,6 +157,10 @@ int hvk_ctx_set_tuning(hvk_ctx* ctx, int param, int value) {
             if (value < 0 || value > 64) return HVK_ERR_ARG;
             ctx->tune_splits = value;
             return HVK_OK;
+        case HVK_TUNE_SHAPE:
+            if (value < 0 || value > 5) return HVK_ERR_ARG;
+            ctx->tune_shape = value;
+            return HVK_OK;
         case HVK_TUNE_SELECT_KIB:
             if (value < 0) return HVK_ERR_ARG;
             ctx->tune_select_kib = value;

@@ -71,6 +71,7 @@ struct hvk_ctx {
     // tuning (hvk_ctx_set_tuning); 0 = automatic
     int tune_splits = 0;
     int tune_select_kib = 0;  // large-k selection workspace budget, 0 = 1 GiB
+    int tune_shape = 0;       // tile shape of the bounded search (8-entry lists)
 };
 
 // Kernel launchers (defined in the .hip files).

@@ -332,11 +332,30 @@ constexpr int kLdsBytes = 160 * 1024;
 constexpr int kMfmaGM = 2, kMfmaGN = 4, kMfmaWaves = kMfmaGM * kMfmaGN;
 constexpr int kMfmaThreads = 64 * kMfmaWaves;
 
-template <int QL, int RL>
-struct MfmaTile {  // wave tile blocks: QL * RL * WX * WY = 8
-    static constexpr int WX = QL == 1 ? 4 : 2;
-    static constexpr int WY = RL == 1 ? 2 : 1;
-    static constexpr int MT = kMfmaGM * 32 * WX, NT = kMfmaGN * 32 * WY;
+// Tile of a launch.  SH = 0: the default, a 2 x 4 wave grid whose wave tiles
+// hold QL * RL * WX * WY = 8 blocks of 32 x 32.  SH > 0 (one limb per side
+// only, the bounded form's shapes for the batch autotuner): other wave grids
+// GM x GN (8 waves) and wave tiles, at most 8 blocks per wave:
+//   SH  GM x GN  WX x WY  queries x references per tile
+//    1  2 x 4    2 x 2    128 x 256
+//    2  2 x 4    4 x 1    256 x 128
+//    3  2 x 4    2 x 4    128 x 512
+//    4  4 x 2    4 x 2    512 x 128
+//    5  4 x 2    2 x 4    256 x 256
+// (Fewer than 128 rows or columns would leave waves without a share of each
+// slab's 16-row loads, and the floor groups need GN <= KP / 2.)
+constexpr int kMfmaShapes = 6;
+template <int QL, int RL, int SH = 0>
+struct MfmaTile {
+    static_assert(SH == 0 || (QL == 1 && RL == 1), "tile shapes are one-limb only");
+    static constexpr int GM = SH == 4 || SH == 5 ? 4 : kMfmaGM;
+    static constexpr int GN = SH == 4 || SH == 5 ? 2 : kMfmaGN;
+    static constexpr int WX = SH == 0 ? (QL == 1 ? 4 : 2)
+                            : (SH == 1 || SH == 3 || SH == 5) ? 2 : 4;
+    static constexpr int WY = SH == 0 ? (RL == 1 ? 2 : 1)
+                            : SH == 1 ? 2 : SH == 2 ? 1 : SH == 4 ? 2 : 4;
+    static_assert(GM * GN == kMfmaWaves, "8 waves");
+    static constexpr int MT = GM * 32 * WX, NT = GN * 32 * WY;
 };
 
 struct TileMap {
@@ -366,22 +385,29 @@ __device__ __forceinline__ int swz(int row, int chunk) {
     return row * 64 + ((chunk ^ swz_key(row)) << 4);
 }
 
-template <int QL, int RL, int KP, int ST>
+template <int QL, int RL, int KP, int ST, int SH = 0>
 struct MfmaSmem {
-    static constexpr int MT = MfmaTile<QL, RL>::MT, NT = MfmaTile<QL, RL>::NT;
+    static constexpr int MT = MfmaTile<QL, RL, SH>::MT, NT = MfmaTile<QL, RL, SH>::NT;
     int8_t q[ST][QL][MT * kMfmaKS];
     int8_t r[ST][RL][NT * kMfmaKS];  // also the candidate buffer of the epilogue
     TopLds<MT, KP> tl;
     float iq[MT];
     float2 bq[MT];
-    // candidates per query and pass that the idle reference stages hold
-    static constexpr int kCap = (int)(sizeof(r) / (MT * 8)) < 32 ? (int)(sizeof(r) / (MT * 8)) : 32;
+    // The epilogue's per-wave score stash (16 floats per lane) sits in the
+    // idle query stages and the candidate buffer in the idle reference
+    // stages; when the query stages are too small (tiles of 128 rows) the
+    // stash starts at the query stages and runs on, and the buffer follows it.
+    static constexpr int kStash = kMfmaWaves * 64 * 16 * (int)sizeof(float);
+    static constexpr bool kStashInQ = (int)sizeof(q) >= kStash;
+    static constexpr int kBufBytes = kStashInQ ? (int)sizeof(r) : (int)(sizeof(q) + sizeof(r)) - kStash;
+    // candidates per query and pass that the buffer holds
+    static constexpr int kCap = kBufBytes / (MT * 8) < 32 ? kBufBytes / (MT * 8) : 32;
 };
 
 // LDS stages: three when they fit beside the top lists, else two.
-template <int QL, int RL, int KP>
+template <int QL, int RL, int KP, int SH = 0>
 constexpr int mfma_stages() {
-    return sizeof(MfmaSmem<QL, RL, KP, 3>) <= (size_t)kLdsBytes ? 3 : 2;
+    return sizeof(MfmaSmem<QL, RL, KP, 3, SH>) <= (size_t)kLdsBytes ? 3 : 2;
 }
 
 // s_waitcnt with vmcnt(n), expcnt and lgkmcnt left free (gfx9 encoding).
@@ -419,7 +445,7 @@ __device__ __forceinline__ void lds_barrier() { asm volatile("s_barrier" ::: "me
 // MAT: no top lists; the exact dot of every pair (limb products combined in
 // int64) gives its FP64 cosine, written as an ord_f64 key to mat[m][n] (row
 // stride nr); qn64 / rn64 are the exact squared norms.
-template <int QL, int RL, int KP, bool BND = false, bool IND = false, bool MAT = false>
+template <int QL, int RL, int KP, bool BND = false, bool IND = false, bool MAT = false, int SH = 0>
 __global__ void __launch_bounds__(kMfmaThreads, 1)
 mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
             const float* __restrict__ iq, int nq, const int8_t* __restrict__ rlo,
@@ -435,15 +461,16 @@ mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
     // after each tile and adopts the largest: a slice's KP-th best key is a
     // lower bound on the KP-th best of the merged lists, so the keys it
     // rejects could not have entered the merged top KP.
-    using T = MfmaTile<QL, RL>;
+    using T = MfmaTile<QL, RL, SH>;
     constexpr int WX = T::WX, WY = T::WY, MT = T::MT, NT = T::NT, KS = kMfmaKS;
-    constexpr int ST = mfma_stages<QL, RL, KP>();
+    constexpr int GM = T::GM, GN = T::GN;
+    constexpr int ST = mfma_stages<QL, RL, KP, SH>();
     constexpr int NTH = kMfmaThreads;
     constexpr int QJ = MT * KS / 1024 / kMfmaWaves;  // wave-loads per wave per Q limb
     constexpr int RJ = NT * KS / 1024 / kMfmaWaves;  // per R limb
     static_assert(QJ >= 1 && RJ >= 1, "every wave requests a share of each operand");
     constexpr int LPS = QL * QJ + RL * RJ;  // wave-loads per wave per slab
-    __shared__ __attribute__((aligned(16))) MfmaSmem<QL, RL, KP, ST> sm;
+    __shared__ __attribute__((aligned(16))) MfmaSmem<QL, RL, KP, ST, SH> sm;
     auto& tl = sm.tl;
 
     // XCD-grouped tile of this workgroup (all branches below are uniform)
@@ -463,7 +490,7 @@ mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
-    const int wm = wave % kMfmaGM, wn = wave / kMfmaGM;
+    const int wm = wave % GM, wn = wave / GM;
     auto qrow = [&](int m) { return IND ? q_rows[m] : m; };
     if constexpr (!MAT) {
         tl.init(tid, NTH);
@@ -635,7 +662,9 @@ mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
         constexpr int NMF = QL * RL * WX * WY;  // MFMAs per K-step
         constexpr int NFR = QL * WX + RL * WY;  // fragment reads per K-step
         constexpr int NSL = NMF / 2;            // slots of two MFMAs
-        static_assert(NMF % 2 == 0 && LPS <= NSL && NFR <= 2 * NSL, "interleave pattern");
+        // (tiles with more requests or fragment reads than that give the extra
+        // ones to the slots again, in turn)
+        static_assert(NMF % 2 == 0 && LPS <= 3 * NSL && NFR <= 3 * NSL, "interleave pattern");
         for (int kk = 0; kk + 1 < nk; ++kk) {
             // past the last slab the request repeats slab nk - 1 into the
             // stage of slab kk - 1, which no later read uses: the loop body
@@ -648,10 +677,16 @@ mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
                 constexpr int i = decltype(I)::value;
                 mfma_one(std::integral_constant<int, 2 * i>{}, fa0, fb0);
                 if constexpr (i < LPS) request_one(I, kreq, nxt, ln);
+                if constexpr (i + NSL < LPS)
+                    request_one(std::integral_constant<int, i + NSL>{}, kreq, nxt, ln);
+                if constexpr (i + 2 * NSL < LPS)
+                    request_one(std::integral_constant<int, i + 2 * NSL>{}, kreq, nxt, ln);
                 if constexpr (2 * i < NFR)
                     frag_one(std::integral_constant<int, 2 * i>{}, fa1, fb1, cur, 1, ln);
                 if constexpr (2 * i + 1 < NFR)
                     frag_one(std::integral_constant<int, 2 * i + 1>{}, fa1, fb1, cur, 1, ln);
+                if constexpr (2 * NSL + i < NFR)
+                    frag_one(std::integral_constant<int, 2 * NSL + i>{}, fa1, fb1, cur, 1, ln);
                 mfma_one(std::integral_constant<int, 2 * i + 1>{}, fa0, fb0);
                 __builtin_amdgcn_sched_barrier(0);
             });
@@ -666,6 +701,8 @@ mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
                 mfma_one(std::integral_constant<int, 2 * i>{}, fa1, fb1);
                 if constexpr (i + NSL < NFR)
                     frag_one(std::integral_constant<int, i + NSL>{}, fa0, fb0, cur, 0, ln);
+                if constexpr (i + 2 * NSL < NFR)
+                    frag_one(std::integral_constant<int, i + 2 * NSL>{}, fa0, fb0, cur, 0, ln);
                 mfma_one(std::integral_constant<int, 2 * i + 1>{}, fa1, fb1);
                 __builtin_amdgcn_sched_barrier(0);
             });
@@ -761,8 +798,8 @@ mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
         if (n0 == n_begin) {
             // (the reference stages hold the floors; the offers below reuse them)
             uint32_t* floor_o = reinterpret_cast<uint32_t*>(&sm.r[0][0][0]);
-            constexpr int GL = 2 * 32 * kMfmaGN / KP;  // lanes per group
-            static_assert(KP % (2 * kMfmaGN) == 0 && GL >= 1 && GL <= 32, "floor groups");
+            constexpr int GL = 2 * 32 * GN / KP;  // lanes per group
+            static_assert(KP % (2 * GN) == 0 && GL >= 1 && GL <= 32, "floor groups");
             for (int i = tid; i < MT; i += NTH) floor_o[i] = 0xFFFFFFFFu;
             __syncthreads();
             static_for<WX>([&](auto X) {
@@ -800,10 +837,14 @@ mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
             }
             __syncthreads();
         }
-        static_assert(sizeof(sm.q) >= kMfmaWaves * 64 * 16 * sizeof(float), "score stash");
+        using Smem = MfmaSmem<QL, RL, KP, ST, SH>;
+        static_assert(Smem::kCap >= 1 && sizeof(sm.q) + sizeof(sm.r) >= (size_t)Smem::kStash,
+                      "score stash and candidate buffer");
         float* stage = reinterpret_cast<float*>(&sm.q[0][0][0]) + wave * 1024 + lane_e;
-        constexpr int CAP = MfmaSmem<QL, RL, KP, ST>::kCap;
-        uint64_t* cbuf = reinterpret_cast<uint64_t*>(&sm.r[0][0][0]);
+        constexpr int CAP = Smem::kCap;
+        uint64_t* cbuf = Smem::kStashInQ
+                             ? reinterpret_cast<uint64_t*>(&sm.r[0][0][0])
+                             : reinterpret_cast<uint64_t*>(&sm.q[0][0][0] + Smem::kStash);
         static_for<WX>([&](auto X) {
             constexpr int x = decltype(X)::value;
             const int mb = wm * 32 * WX + x * 32 + 4 * (lane_e >> 5);
@@ -1335,15 +1376,28 @@ int run_mfma(hvk_ctx* ctx, const int8_t* qlo, const int8_t* qhi, const int64_t* 
                                 b, pl.map.n_splits, qn, nq, rn, nr, k, idx, score, cosv, n_res, st);
 }
 
-// Reference slices and tile map of a bounded one-limb search.
-inline MfmaPlan plan_bounded(hvk_ctx* ctx, int nq, int nr) {
-    using T = MfmaTile<1, 1>;
-    return plan_mfma(ctx, (nq + T::MT - 1) / T::MT, (nr + T::NT - 1) / T::NT, 0);
+// Rows and columns of the one-limb tile of shape sh (0 .. kMfmaShapes - 1).
+inline void shape_tile(int sh, int& mt, int& nt) {
+    switch (sh) {
+        case 1: mt = MfmaTile<1, 1, 1>::MT; nt = MfmaTile<1, 1, 1>::NT; return;
+        case 2: mt = MfmaTile<1, 1, 2>::MT; nt = MfmaTile<1, 1, 2>::NT; return;
+        case 3: mt = MfmaTile<1, 1, 3>::MT; nt = MfmaTile<1, 1, 3>::NT; return;
+        case 4: mt = MfmaTile<1, 1, 4>::MT; nt = MfmaTile<1, 1, 4>::NT; return;
+        case 5: mt = MfmaTile<1, 1, 5>::MT; nt = MfmaTile<1, 1, 5>::NT; return;
+        default: mt = MfmaTile<1, 1>::MT; nt = MfmaTile<1, 1>::NT; return;
+    }
+}
+
+// Reference slices and tile map of a bounded one-limb search on tile shape sh.
+inline MfmaPlan plan_bounded(hvk_ctx* ctx, int nq, int nr, int sh = 0) {
+    int mt, nt;
+    shape_tile(sh, mt, nt);
+    return plan_mfma(ctx, (nq + mt - 1) / mt, (nr + nt - 1) / nt, 0);
 }
 
 // The launches of a bounded search on buffers already laid out (b sized by
-// search_sizes(nq, nr, pl.map.n_splits, KP)).
-template <int KP>
+// search_sizes(nq, nr, pl.map.n_splits, KP)), on tile shape SH.
+template <int KP, int SH = 0>
 int launch_mfma_bounded(hvk_ctx* ctx, const MfmaPlan& pl, const hvk_bundles& q,
                         const hvk_bundles& r, int64_t ld, int k, const SearchBufs& b, int32_t* idx,
                         float* score, double* cosv, int32_t* n_res, hipStream_t st) {
@@ -1351,9 +1405,10 @@ int launch_mfma_bounded(hvk_ctx* ctx, const MfmaPlan& pl, const hvk_bundles& q,
     launch_inv_norm(q.sqnorm, nq, r.sqnorm, nr, b,
                     BoundRows{q.maxabs, q.resid1, r.maxabs, r.resid1}, true, st);
     timing_mark(ctx, 1, false, st);
-    hipLaunchKernelGGL((mfma_kernel<1, 1, KP, true>), dim3(pl.blocks), dim3(kMfmaThreads), 0, st,
-                       q.clamp8, nullptr, b.iq, nq, r.clamp8, nullptr, b.ir, nr, ld, pl.map,
-                       b.part, b.bq, b.br, nullptr, nullptr, b.qthr);
+    hipLaunchKernelGGL((mfma_kernel<1, 1, KP, true, false, false, SH>), dim3(pl.blocks),
+                       dim3(kMfmaThreads), 0, st, q.clamp8, nullptr, b.iq, nq, r.clamp8, nullptr,
+                       b.ir, nr, ld, pl.map, b.part, b.bq, b.br, nullptr, nullptr, b.qthr, nullptr,
+                       nullptr, nullptr);
     timing_mark(ctx, 1, true, st);
     HVK_HIP(hipGetLastError());
     const RowsLimb rows{q.lo, q.hi, r.lo, r.hi, ld};
@@ -1363,12 +1418,24 @@ int launch_mfma_bounded(hvk_ctx* ctx, const MfmaPlan& pl, const hvk_bundles& q,
     return exact_pass2(ctx, q, r, ld, k, b, idx, score, cosv, n_res, st);
 }
 
+// Tile shapes other than the default are compiled for 8-entry lists (top-5).
 template <int KP>
 int run_mfma_bounded(hvk_ctx* ctx, const hvk_bundles& q, const hvk_bundles& r, int64_t ld, int k,
                      int32_t* idx, float* score, double* cosv, int32_t* n_res, hipStream_t st) {
-    const MfmaPlan pl = plan_bounded(ctx, q.n, r.n);
+    const int sh = KP == 8 ? ctx->tune_shape : 0;
+    const MfmaPlan pl = plan_bounded(ctx, q.n, r.n, sh);
     SearchBufs b;
     if (int rc = prepare(ctx, q.n, r.n, pl.map.n_splits, KP, b)) return rc;
+    if constexpr (KP == 8) {
+        switch (sh) {
+            case 1: return launch_mfma_bounded<KP, 1>(ctx, pl, q, r, ld, k, b, idx, score, cosv, n_res, st);
+            case 2: return launch_mfma_bounded<KP, 2>(ctx, pl, q, r, ld, k, b, idx, score, cosv, n_res, st);
+            case 3: return launch_mfma_bounded<KP, 3>(ctx, pl, q, r, ld, k, b, idx, score, cosv, n_res, st);
+            case 4: return launch_mfma_bounded<KP, 4>(ctx, pl, q, r, ld, k, b, idx, score, cosv, n_res, st);
+            case 5: return launch_mfma_bounded<KP, 5>(ctx, pl, q, r, ld, k, b, idx, score, cosv, n_res, st);
+            default: break;
+        }
+    }
     return launch_mfma_bounded<KP>(ctx, pl, q, r, ld, k, b, idx, score, cosv, n_res, st);
 }
 

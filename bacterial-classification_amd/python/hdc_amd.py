@@ -43,6 +43,7 @@ BATCH_FORM_TWO_LIMB = 1
 
 TUNE_SPLITS = 1
 TUNE_SELECT_KIB = 2
+TUNE_SHAPE = 3
 
 
 class HvkError(RuntimeError):
@@ -148,12 +149,13 @@ def lib():
     L.hvk_db_upload.argtypes = [vp, vp, ctypes.POINTER(EncodeOut)]
     L.hvk_batch_create.argtypes = [i32, i32, i32, ctypes.POINTER(vp)]
     L.hvk_batch_destroy.argtypes = [vp]
-    L.hvk_batch_config.argtypes = [vp, i32, i32, i32]
-    L.hvk_batch_get_config.argtypes = [vp] + [ctypes.POINTER(ctypes.c_int)] * 4
+    L.hvk_batch_config.argtypes = [vp, i32, i32, i32, i32]
+    L.hvk_batch_get_config.argtypes = [vp] + [ctypes.POINTER(ctypes.c_int)] * 5
     L.hvk_batch_classify.argtypes = [vp, vp, vp, vp, i32, ctypes.POINTER(Bundles), i32, vp, vp,
                                      vp, vp, vp]
     L.hvk_batch_autotune.argtypes = [vp, vp, vp, vp, i32, ctypes.POINTER(Bundles), i32, vp, vp,
-                                     i32, ctypes.POINTER(ctypes.c_float), vp]
+                                     i32, ctypes.POINTER(ctypes.c_float), vp,
+                                     ctypes.POINTER(ctypes.c_int), vp]
     _lib = L
     return L
 
@@ -625,14 +627,14 @@ class Batch:
         except Exception:
             pass
 
-    def config(self, form=BATCH_FORM_BOUNDED, splits=0, chunk=262144):
-        check(lib().hvk_batch_config(self.h, form, splits, chunk), "hvk_batch_config")
+    def config(self, form=BATCH_FORM_BOUNDED, splits=0, chunk=262144, shape=0):
+        check(lib().hvk_batch_config(self.h, form, shape, splits, chunk), "hvk_batch_config")
 
     def get_config(self):
-        v = [ctypes.c_int() for _ in range(4)]
+        v = [ctypes.c_int() for _ in range(5)]
         check(lib().hvk_batch_get_config(self.h, *[ctypes.byref(x) for x in v]),
               "hvk_batch_get_config")
-        return dict(zip(("form", "splits", "chunk", "graphs"), (x.value for x in v)))
+        return dict(zip(("form", "shape", "splits", "chunk", "graphs"), (x.value for x in v)))
 
     def classify(self, seqs: DeviceSeqs, db: Bundles, k: int, out: TopK | None = None) -> TopK:
         o = out or _topk_bufs(seqs.n, k)
@@ -642,12 +644,22 @@ class Batch:
                                        _ptr(o.n_rescored), _stream()), "hvk_batch_classify")
         return o
 
-    def autotune(self, seqs: DeviceSeqs, db: Bundles, k: int, out: TopK, sample: int) -> float:
+    def autotune(self, seqs: DeviceSeqs, db: Bundles, k: int, out: TopK, sample: int,
+                 table: list | None = None) -> float:
+        """Applies the fastest setting; returns its ms per million reads.  With
+        `table` (a list), appends one dict per candidate timed."""
         ms = ctypes.c_float()
+        rows = ctypes.c_int(64)
+        tab = np.zeros((64, 5), np.float32)
         check(lib().hvk_batch_autotune(self.h, _ptr(seqs.packed), _ptr(seqs.invalid),
                                        seqs.host_offsets.ctypes.data, seqs.n, ctypes.byref(db), k,
                                        _ptr(out.idx), _ptr(out.score), sample, ctypes.byref(ms),
-                                       _stream()), "hvk_batch_autotune")
+                                       tab.ctypes.data, ctypes.byref(rows), _stream()),
+              "hvk_batch_autotune")
+        if table is not None:
+            for f, sh, sp, ch, t in tab[:rows.value]:
+                table.append({"form": ["bounded", "two_limb"][int(f)], "shape": int(sh),
+                              "splits": int(sp), "chunk": int(ch), "ms_per_mread": float(t)})
         return ms.value
 
 

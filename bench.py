@@ -614,7 +614,8 @@ def run_batch(a):
     b = H.Batch(DIM, KMER, 524288)
     out = H._topk_bufs(nq, TOPK)
     t1 = time.time()
-    ms_per_m = b.autotune(seqs, dbb, TOPK, out, a.batch_sample)
+    table = []
+    ms_per_m = b.autotune(seqs, dbb, TOPK, out, a.batch_sample, table=table)
     cfg = b.get_config()
     print(f"[batch] autotune {time.time() - t1:.1f}s: {cfg} ({ms_per_m:.0f} ms per M reads)",
           file=sys.stderr, flush=True)
@@ -662,9 +663,10 @@ def run_batch(a):
         "config": {"workload": f"config 3: D={DIM}, k={KMER}, {n_refs} refs, {nq} reads "
                                f"log-uniform 100-1600 nt (mean {lens.mean():.0f}), 1% error, "
                                f"top-{TOPK}", "reads": nq, "refs": n_refs, "topk": TOPK},
-        "autotune": {"form": ["bounded", "two_limb"][cfg["form"]], "splits": cfg["splits"],
-                     "chunk": cfg["chunk"], "ms_per_mread_sample": ms_per_m,
-                     "sample": a.batch_sample},
+        "autotune": {"form": ["bounded", "two_limb"][cfg["form"]], "shape": cfg["shape"],
+                     "splits": cfg["splits"], "chunk": cfg["chunk"],
+                     "ms_per_mread_sample": ms_per_m, "sample": a.batch_sample,
+                     "table": table},
         "graphs": b.get_config()["graphs"],
         "direct_path": {"reads": nd, "reads_per_s": nd / direct_s, "top5_equal_batch": same},
         "n_rescored": int(out.n_rescored.item()),

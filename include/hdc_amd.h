@@ -75,9 +75,14 @@ int hvk_ctx_last_timing(hvk_ctx* ctx, float* encode_ms, float* search_ms);
  * HVK_TUNE_SPLITS: number of reference slices per query tile of the MFMA
  * search, 0 = automatic (the default), else 1..64.
  * HVK_TUNE_SELECT_KIB: workspace budget (KiB) of the selection path for
- * k > 29, which bounds how many queries one chunk holds; 0 = 1 GiB. */
+ * k > 29, which bounds how many queries one chunk holds; 0 = 1 GiB.
+ * HVK_TUNE_SHAPE: tile of hvk_search_mfma_bounded for k <= 5, as queries x
+ * references per workgroup: 0 = 256 x 256 (default), 1 = 128 x 256,
+ * 2 = 256 x 128, 3 = 128 x 512, 4 = 512 x 128, 5 = 256 x 256 on a 4 x 2
+ * wave grid. */
 #define HVK_TUNE_SPLITS 1
 #define HVK_TUNE_SELECT_KIB 2
+#define HVK_TUNE_SHAPE 3
 int hvk_ctx_set_tuning(hvk_ctx* ctx, int param, int value);
 
 /* ---- packing (host) -----------------------------------------------------
@@ -358,9 +363,10 @@ int hvk_db_upload(const hvk_db* db, const hvk_ctx* ctx, const hvk_encode_out* ds
  *    hipGraph; later calls replay it, whatever their read and output
  *    buffers.  At most 64 graphs are kept (least recently used dropped);
  *  - hvk_batch_autotune times candidate settings (search form: bounded one
- *    limb on 256x256 tiles or two limbs on 256x128 / 128x128 tiles;
- *    reference slices per read tile; chunk size) on a sample of the batch and
- *    keeps the fastest.  Results never depend on the settings.
+ *    limb or two limbs; the bounded form's tile shape, reads x references
+ *    per workgroup from 128 to 512 each, see HVK_TUNE_SHAPE; reference
+ *    slices per read tile; chunk size) on a sample of the batch and keeps the
+ *    fastest.  Results never depend on the settings.
  * A batch owns a private context for D, k, whose workspaces back the
  * captured graphs; `db` may come from any context of the same D, k and is
  * read, not copied: it must stay valid and unchanged while graphs that use
@@ -370,10 +376,13 @@ typedef struct hvk_batch hvk_batch;
 #define HVK_BATCH_FORM_TWO_LIMB 1
 int hvk_batch_create(int dim, int kmer, int chunk, hvk_batch** out);
 int hvk_batch_destroy(hvk_batch* b);
-/* form: HVK_BATCH_FORM_*; splits: 0 = automatic, else 1..64; chunk >= 256.
- * Changing a setting drops the captured graphs. */
-int hvk_batch_config(hvk_batch* b, int form, int splits, int chunk);
-int hvk_batch_get_config(const hvk_batch* b, int* form, int* splits, int* chunk, int* n_graphs);
+/* form: HVK_BATCH_FORM_*; shape: the bounded form's tile for top-5 (the
+ * values of HVK_TUNE_SHAPE, 0..5; other k use shape 0); splits: 0 =
+ * automatic, else 1..64; chunk >= 256.  Changing a setting drops the
+ * captured graphs. */
+int hvk_batch_config(hvk_batch* b, int form, int shape, int splits, int chunk);
+int hvk_batch_get_config(const hvk_batch* b, int* form, int* shape, int* splits, int* chunk,
+                         int* n_graphs);
 /* d_packed / d_invalid: hvk_pack layout of all reads (device); h_offsets:
  * n+1 base offsets (host).  Outputs [n][k] (device); d_cos64 and
  * d_n_rescored (the number of reads re-scored by the exact passes, summed
@@ -383,17 +392,20 @@ int hvk_batch_classify(hvk_batch* b, const uint32_t* d_packed, const uint32_t* d
                        const int64_t* h_offsets, int n, const hvk_bundles* db, int k,
                        int32_t* d_idx, float* d_score, double* d_cos64, int32_t* d_n_rescored,
                        hvk_stream stream);
-/* Times each candidate setting on the first min(sample, n) reads of the
- * batch (one direct run, then two timed graph replays, the faster counted,
- * into the caller's output buffers) and applies the fastest; *ms_per_mread
- * (nullable) gets its time per million reads.  Candidates: the bounded form
- * with 4, 8 or 16 reference slices at chunks of 1/4, 1/2 and all of the
- * sample (at least 256 reads), and the two-limb form.  Synchronises
- * `stream`. */
+/* Times candidate settings on the first min(sample, n) reads of the batch
+ * (one direct run, then two timed graph replays, the faster counted, into
+ * the caller's output buffers) and applies the fastest; *ms_per_mread
+ * (nullable) gets its time per million reads.  Coordinate search: every
+ * tile shape (top-5) at automatic slices and half the sample per chunk, then
+ * 4, 8 and 16 slices on the best shape, then chunks of 1/4 and all of the
+ * sample (at least 256 reads), and the two-limb form.  table (nullable,
+ * *table_rows rows of 5 floats on entry): one row per candidate timed --
+ * form, shape, splits, chunk, ms per million reads; *table_rows is set to
+ * the rows written.  Synchronises `stream`. */
 int hvk_batch_autotune(hvk_batch* b, const uint32_t* d_packed, const uint32_t* d_invalid,
                        const int64_t* h_offsets, int n, const hvk_bundles* db, int k,
                        int32_t* d_idx, float* d_score, int sample, float* ms_per_mread,
-                       hvk_stream stream);
+                       float* table, int* table_rows, hvk_stream stream);
 
 /* ---- synthetic workloads (host, deterministic) -------------------------
  * Ancestors of uniform ACGT, descendants by substitution + indels, and reads

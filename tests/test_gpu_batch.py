@@ -57,15 +57,17 @@ def test_direct_path_matches_oracle(work):
     assert np.array_equal(direct.idx.cpu().numpy(), ref_idx)
 
 
-@pytest.mark.parametrize("chunk,form,splits", [(256, 0, 0), (512, 0, 2), (1000, 0, 8),
-                                               (4096, 0, 0), (700, 1, 4)])
-def test_batch_equals_direct(work, chunk, form, splits):
+@pytest.mark.parametrize("chunk,form,splits,shape", [(256, 0, 0, 0), (512, 0, 2, 0), (1000, 0, 8, 0),
+                                                     (4096, 0, 0, 0), (700, 1, 4, 0), (512, 0, 0, 1),
+                                                     (768, 0, 2, 2), (512, 0, 0, 3), (1024, 0, 4, 4),
+                                                     (600, 0, 0, 5)])
+def test_batch_equals_direct(work, chunk, form, splits, shape):
     import hdc_amd as H
     import torch
     ctx, db, seqs, q, direct = work
-    print(f"batch chunk={chunk} form={form} splits={splits} n={seqs.n}", flush=True)
+    print(f"batch chunk={chunk} form={form} splits={splits} shape={shape} n={seqs.n}", flush=True)
     b = H.Batch(DIM, K, chunk)
-    b.config(form, splits, chunk)
+    b.config(form, splits, chunk, shape)
     dbb = H.bundles(db)
     out = b.classify(seqs, dbb, TOPK)  # direct runs + captures
     _same(out, direct)
@@ -136,10 +138,13 @@ def test_batch_autotune(work):
     b = H.Batch(DIM, K, 512)
     dbb = H.bundles(db)
     out = H._topk_bufs(seqs.n, TOPK)
-    ms = b.autotune(seqs, dbb, TOPK, out, sample=1024)
+    table = []
+    ms = b.autotune(seqs, dbb, TOPK, out, sample=1024, table=table)
     cfg = b.get_config()
     print("autotune", cfg, ms, flush=True)
     assert ms > 0 and cfg["chunk"] >= 256
+    assert {r["shape"] for r in table} == set(range(6))  # every tile shape was timed
+    assert min(r["ms_per_mread"] for r in table) == pytest.approx(ms)
     out = b.classify(seqs, dbb, TOPK, out=out)
     _same(out, direct)
     b.close()

@@ -285,3 +285,23 @@ def test_large_k_chunked(workload, form):
         _check(res, ref_idx, ref_cos, 64)
     finally:
         ctx.set_tuning(H.TUNE_SELECT_KIB, 0)
+
+
+@pytest.mark.parametrize("shape", [1, 2, 3, 4, 5])
+def test_bounded_tile_shapes(workload, shape):
+    """Every tile shape of the bounded form (HVK_TUNE_SHAPE) gives the oracle's
+    top-5, with exact ties and the second pass (41 copies) included."""
+    import hdc_amd as H
+    ctx, refs, reads, rc, qc, w = workload
+    ctx.set_tuning(H.TUNE_SHAPE, shape)
+    try:
+        print(f"bounded shape={shape}", flush=True)
+        ref_idx, ref_cos = orc.topk16(qc, rc, 5)
+        _check(H.search_mfma_bounded(ctx, reads, refs, 5), ref_idx, ref_cos, 5)
+        big = np.concatenate([rc[:30], np.repeat(rc[5:6], 40, axis=0), rc[30:60]])
+        qn = np.concatenate([rc[5:6], qc[:7], rc[5:6]])
+        ref_idx, ref_cos = orc.topk16(qn, big, 5)
+        _check(H.search_mfma_bounded(ctx, encoded_from_counts(qn, ctx.ld),
+                                     encoded_from_counts(big, ctx.ld), 5), ref_idx, ref_cos, 5)
+    finally:
+        ctx.set_tuning(H.TUNE_SHAPE, 0)
