@@ -28,7 +28,9 @@
 namespace hvk {
 namespace {
 
-constexpr int kDescG = 8;  // candidates scored per pass over a read row
+// candidates scored per pass over a read row: 5 when no candidate list is
+// longer (the registers of 5 fit five waves per SIMD), else 8 (four waves)
+constexpr int kDescGSmall = 5, kDescG = 8;
 constexpr int kDescThreads = 256;
 constexpr int kMaxBeam = 16;
 constexpr int kMaxLevels = 32;
@@ -240,11 +242,11 @@ __device__ __forceinline__ int32_t sdot2(int32_t a, int32_t b, int32_t c) {
 // step, so each load instruction reads 1 KB of one row (one page, fully
 // coalesced) -- a lane-per-pair layout would touch 64 rows, and at the deep
 // levels 64 pages, per instruction.  The wave walks the distinct categories
-// of its pairs (wave-uniform), kDescG children at a time; each child slab is
+// of its pairs (wave-uniform), G children at a time; each child slab is
 // loaded once for all the wave's pairs of that category.  The per-lane int64
 // sums are reduced across the wave at the end of each pass.
-// pairs per wave: 4 leaves ~120 VGPRs and four waves per SIMD, which beat
-// 8 pairs (one child load shared by more pairs, two waves per SIMD) and 2
+// pairs per wave: 4 leaves ~120 VGPRs (G = 8) and four waves per SIMD, which
+// beat 8 pairs (one child load shared by more pairs, two waves per SIMD) and 2
 constexpr int kDescP = 4;
 constexpr int kDescStep = 64 * 8;         // elements per wave step
 
@@ -254,8 +256,8 @@ __device__ __forceinline__ int64_t wave_sum_i64(int64_t v) {
     return v;
 }
 
-template <int E>
-__global__ void __launch_bounds__(kDescThreads)
+template <int E, int GW>
+__global__ void __launch_bounds__(kDescThreads, GW <= kDescGSmall ? 5 : 4)
 desc_score(TreeDev t, const int16_t* __restrict__ q, int64_t qld, const int64_t* __restrict__ qn,
            int B, int64_t K, const int32_t* __restrict__ pairs, const int32_t* __restrict__ pkey,
            const int32_t* __restrict__ total, int Fmax, float* __restrict__ sc) {
@@ -287,22 +289,22 @@ desc_score(TreeDev t, const int16_t* __restrict__ q, int64_t qld, const int64_t*
         const int16_t* qrow[kDescP];
 #pragma unroll
         for (int p = 0; p < kDescP; ++p) qrow[p] = q + (int64_t)(pr[p] / B) * qld;
-        for (int g0 = 0; g0 < F; g0 += kDescG) {
-            const int G = min(kDescG, F - g0);
-            const int16_t* lo[kDescG];
-            const int16_t* hi[kDescG];
+        for (int g0 = 0; g0 < F; g0 += GW) {
+            const int G = min(GW, F - g0);
+            const int16_t* lo[GW];
+            const int16_t* hi[GW];
 #pragma unroll
-            for (int g = 0; g < kDescG; ++g) {
+            for (int g = 0; g < GW; ++g) {
                 const int c = list[g0 + (g < G ? g : G - 1)];
                 lo[g] = t.lo + (int64_t)c * t.ld;
                 hi[g] = t.hi + (int64_t)c * t.ld;
             }
-            int64_t acc[kDescP][kDescG];
-            int32_t pl[kDescP][kDescG], ph[kDescP][kDescG];
+            int64_t acc[kDescP][GW];
+            int32_t pl[kDescP][GW], ph[kDescP][GW];
 #pragma unroll
             for (int p = 0; p < kDescP; ++p)
 #pragma unroll
-                for (int g = 0; g < kDescG; ++g) { acc[p][g] = 0; pl[p][g] = 0; ph[p][g] = 0; }
+                for (int g = 0; g < GW; ++g) { acc[p][g] = 0; pl[p][g] = 0; ph[p][g] = 0; }
             for (int64_t k = 8 * lane; k < K; k += kDescStep) {
                 int4 qv[kDescP];
 #pragma unroll
@@ -310,7 +312,7 @@ desc_score(TreeDev t, const int16_t* __restrict__ q, int64_t qld, const int64_t*
                     qv[p] = ((mine >> p) & 1) ? *reinterpret_cast<const int4*>(qrow[p] + k)
                                                : make_int4(0, 0, 0, 0);
 #pragma unroll
-                for (int g = 0; g < kDescG; ++g) {
+                for (int g = 0; g < GW; ++g) {
                     if (g >= G) continue;  // uniform
                     const int4 lv = *reinterpret_cast<const int4*>(lo[g] + k);
                     const int4 hv = *reinterpret_cast<const int4*>(hi[g] + k);
@@ -340,7 +342,7 @@ desc_score(TreeDev t, const int16_t* __restrict__ q, int64_t qld, const int64_t*
 #pragma unroll
             for (int p = 0; p < kDescP; ++p)
 #pragma unroll
-                for (int g = 0; g < kDescG; ++g) {
+                for (int g = 0; g < GW; ++g) {
                     if (g < G && ((mine >> p) & 1)) {
                         const int64_t dot = wave_sum_i64(acc[p][g]);
                         if (lane == 0) {
@@ -784,12 +786,19 @@ extern "C" int hvk_descend(hvk_ctx* ctx, const hvk_tree* tree, const int16_t* d_
                                scan_part, cursor);
             hipLaunchKernelGGL(desc_scatter, dim3(rb), dim3(kDescThreads), 0, st, bm[cur], n_q, B,
                                n_cat, cursor, pairs, pkey);
-            if (E == 8)
-                hipLaunchKernelGGL(desc_score<8>, dim3(pb), dim3(kDescThreads), 0, st, t,
-                                   d_q_counts, q_ld, d_q_sqnorm, B, K, pairs, pkey, total, Fmax, sc);
-            else
-                hipLaunchKernelGGL(desc_score<2>, dim3(pb), dim3(kDescThreads), 0, st, t,
-                                   d_q_counts, q_ld, d_q_sqnorm, B, K, pairs, pkey, total, Fmax, sc);
+            auto score = [&](auto kernel) {
+                hipLaunchKernelGGL(kernel, dim3(pb), dim3(kDescThreads), 0, st, t, d_q_counts, q_ld,
+                                   d_q_sqnorm, B, K, pairs, pkey, total, Fmax, sc);
+            };
+            // the longest candidate list of this level: the roots at level 0
+            const bool narrow = (lvl == 0 ? tree->n_roots : max_span) <= kDescGSmall;
+            if (E == 8) {
+                if (narrow) score(desc_score<8, kDescGSmall>);
+                else score(desc_score<8, kDescG>);
+            } else {
+                if (narrow) score(desc_score<2, kDescGSmall>);
+                else score(desc_score<2, kDescG>);
+            }
         }
         if (sel_lds)
             hipLaunchKernelGGL(desc_select, dim3((n_q + kSelThreads - 1) / kSelThreads),

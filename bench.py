@@ -8,14 +8,20 @@ before the timed region; the reference database is encoded once, outside it.
 
 Also measured in the same run: the plain (VALU) form on a subset of reads,
 the main-kernel durations (HIP events on the launch stream) for the roofline,
-and the CPU baseline (the C restatement of the reference's bit-packed path,
-timed on this host) on a bounded sample, whose top-5 must equal the GPU's.
+the CPU baseline (the C restatement of the reference's bit-packed path,
+timed on this host) on a bounded sample, whose top-5 must equal the GPU's,
+the board's power and graphics clock on two untimed steps (amd-smi), and
+the vendor library's int8 GEMM (torch._int_mm on hipBLASLt) at K = D on the
+same card, as the rate this card sustains for dense int8 MFMA work.
 """
 import argparse
 import json
 import os
 import re
+import shutil
+import subprocess
 import sys
+import threading
 import time
 
 import numpy as np
@@ -106,6 +112,83 @@ def cpu_rows(w, n_take, db_host, dbn, n_refs, dim, kmer, topk, gpu_idx, cpu_seco
             "top5_equal_gpu": bool(a["topk_equal_gpu"] and rows["cpu_1"]["topk_equal_gpu"])}
 
 
+class PowerSampler:
+    """Samples the board's socket power and graphics clocks (amd-smi metric)
+    about once a second on a host thread while GPU work runs (kept out of
+    timed regions: a query reaches the card's power controller).  Without
+    amd-smi it records nothing."""
+
+    def __init__(self):
+        self.samples = []
+        self._stop = threading.Event()
+        self._t = None
+
+    def __enter__(self):
+        if shutil.which("amd-smi"):
+            self._t = threading.Thread(target=self._run, daemon=True)
+            self._t.start()
+        return self
+
+    def _run(self):
+        while not self._stop.is_set():
+            try:
+                out = subprocess.run(["amd-smi", "metric", "-p", "-c"], capture_output=True,
+                                     text=True, timeout=10).stdout
+            except (OSError, subprocess.SubprocessError):
+                return
+            p = re.search(r"SOCKET_POWER:\s*([\d.]+)\s*W", out)
+            c = re.findall(r"GFX_\d+:\s*\n\s*CLK:\s*([\d.]+)\s*MHz", out)
+            if p and c:
+                self.samples.append((float(p.group(1)), float(np.mean([float(x) for x in c]))))
+            self._stop.wait(1.0)
+
+    def __exit__(self, *exc):
+        self._stop.set()
+        if self._t:
+            self._t.join(timeout=20)
+
+    def summary(self):
+        s = self.samples[1:] if len(self.samples) > 2 else self.samples  # skip the ramp
+        if not s:
+            return None
+        return {"socket_power_w_median": float(np.median([x[0] for x in s])),
+                "gfx_clk_mhz_median": float(np.median([x[1] for x in s])),
+                "samples": len(s), "source": "amd-smi metric -p -c, ~1/s, untimed repeat of the same work"}
+
+
+def library_int8_gemm(m=16384, n=16384, k=DIM, seconds=3.0):
+    """int8 x int8 -> int32 GEMM through torch._int_mm (hipBLASLt on this
+    card), operands uniform in [-40, 40] like clamped counts, timed with CUDA
+    events over ~`seconds` of back-to-back calls; one 64 x 64 block is
+    checked against int64 CPU arithmetic."""
+    import torch
+    g = torch.Generator(device="cuda").manual_seed(1)
+    a = torch.randint(-40, 41, (m, k), dtype=torch.int8, device="cuda", generator=g)
+    b = torch.randint(-40, 41, (n, k), dtype=torch.int8, device="cuda", generator=g).t()
+    c = torch._int_mm(a, b)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    c = torch._int_mm(a, b)
+    torch.cuda.synchronize()
+    it = max(3, int(seconds / max(time.time() - t0, 1e-4)))
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(it):
+        c = torch._int_mm(a, b)
+    e.record()
+    torch.cuda.synchronize()
+    ms = s.elapsed_time(e) / it
+    with PowerSampler() as pw:  # power and clock on as many calls again, untimed
+        for _ in range(it):
+            c = torch._int_mm(a, b)
+        torch.cuda.synchronize()
+    exact = bool(torch.equal(a[:64].cpu().long() @ b[:, :64].cpu().long(), c[:64, :64].cpu().long()))
+    del a, b, c
+    return {"op": "torch._int_mm (hipBLASLt), int8 x int8 -> int32", "m": m, "n": n, "k": k,
+            "calls": it, "ms_per_call": ms, "tops": 2.0 * m * n * k / (ms / 1000.0) / 1e12,
+            "exact": exact, "power": pw.summary()}
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,6 +203,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-two-limb", action="store_true")
+    ap.add_argument("--no-gemm-ref", action="store_true",
+                    help="skip the library int8 GEMM reference point")
     ap.add_argument("--workload", choices=["classify", "precision", "descent", "batch", "config0"],
                     default="classify",
                     help="classify: the reads/s line (1M reads x 100k refs, top-5); config0: D=8192, "
@@ -246,6 +331,11 @@ def main():
         return (time.time() - t_start) / steps, enc_ms, srch_ms
 
     sec_step, enc_ms, srch_ms = timed(step_bounded, "mfma-1limb-bounded", a.steps, a.warmup)
+    # power and clock on two more (untimed) steps: the sampler stays out of the timed region
+    with PowerSampler() as power:
+        for _ in range(2):
+            step_bounded()
+            torch.cuda.synchronize()
     ms_per_step = 1000.0 * sec_step
     value = nq / sec_step
     n_res = int(out.n_rescored.item())
@@ -267,6 +357,14 @@ def main():
     # packed input + lo/hi/clamped rows + per-row norm, max, residual, k-mer count
     enc_bytes = float(np.sum((lens + 3) // 4)) + nq * (3.0 * ld) + nq * 20
     enc_gbs = enc_bytes / e_kernel / 1e9
+
+    # the vendor library's int8 GEMM on this card, for scale
+    gemm = None
+    if not a.no_gemm_ref:
+        gemm = library_int8_gemm()
+        gemm["search_tops_over_library"] = achieved_tops / gemm["tops"]
+        print(f"[bench] library int8 GEMM: {gemm['tops']:.0f} TOPS (search kernel "
+              f"{achieved_tops:.0f} TOPS)", file=sys.stderr, flush=True)
 
     # plain form on a subset: median of plain_runs timed runs
     plain = None
@@ -322,7 +420,9 @@ def main():
                      "traffic_source": "rocprofv3 --pmc FETCH_SIZE x2.0 + WRITE_SIZE x1.0 "
                                        "(copy-calibrated), profiles/mfma_kernel_config1_pmc.md",
                      "kernel": bounded_kernel,
-                     "kernel_ms": float(np.mean(srch_ms))},
+                     "kernel_ms": float(np.mean(srch_ms)),
+                     "power": power.summary(),
+                     "library_int8_gemm": gemm},
         "encode": encode_line(enc_ms, enc_gbs, nq, lens),
         "mfma_two_limb": two,
         "plain": plain,

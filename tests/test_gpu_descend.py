@@ -154,8 +154,35 @@ def test_descend_matches_oracle(synth, beam, thr):
         assert (ex == 0).sum() > n // 2
 
 
+def test_descend_wide_candidate_lists_match_oracle():
+    """Candidate lists of 10 (wider than the 5 of a five-wave pass): the
+    8-wide passes, two per list, give the oracle's paths and scores."""
+    import hdc_amd as H
+    import torch
+    D, k = 10240, 12
+    sizes = (2, 20, 40, 80, 160, 1600)  # spans 10, 2, 2, 2, 10
+    tax, leaf_seqs, reads = _synthetic(sizes, 1500, 96, seed=11)
+    assert np.diff(np.asarray(tax.child_off)).max() == 10
+    ctx = H.Context(D, k)
+    leaf_enc = H.encode(ctx, _dev_seqs(H, leaf_seqs), counts=True, limbs=False, hi=False)
+    tree = H.build_prototypes(ctx, tax, leaf_enc.counts, tax.ranks[-1])
+    q = H.encode(ctx, _dev_seqs(H, reads), counts=True, limbs=False, hi=False)
+    torch.cuda.synchronize()
+    proto = _oracle_protos(D, k, tax, leaf_seqs)
+    qc = q.counts.cpu().numpy()[:, :D]
+    for beam in (3, 5):
+        print(f"descend wide lists beam={beam}", flush=True)
+        res = H.descend(ctx, tree, q, beam=beam, threshold=-1.0, levels=6)
+        torch.cuda.synchronize()
+        ref = orc.descend(proto, tax.child_off, tax.child_idx, tax.roots, qc, beam, -1.0, 6)
+        _assert_same((res.path.cpu().numpy(), res.score.cpu().numpy(),
+                      res.length.cpu().numpy(), res.exit_level.cpu().numpy()), ref)
+    ctx.close()
+
+
 def test_descend_narrow_partial_sums(synth):
-    """Claiming larger |q| (up to 20000) forces the 2-element int32 partials: same results."""
+    """Claiming larger |q| (5000, 20000) forces the 8- and then the 2-element int32
+    partials: same results."""
     base = _run_both(synth, 3, 0.0)
     for qm in (5000, 20000):
         got, ref = _run_both(synth, 3, 0.0, q_maxabs=qm)
