@@ -60,12 +60,16 @@ struct TreeDev {
     const int32_t* child_idx;
     const int32_t* roots;
     int n_roots;
+    const int32_t* row_maxabs;  // [n_cat] max |p|, nullable
 };
 
-// src/HV.cpp:236-241 on the exact integer dot and norms
+// src/HV.cpp:236-241 on the exact integer dot and norms, as a candidate's
+// score: identify starts every candidate at -1.0f and keeps the max of that
+// and its similarity (targets/identify.cpp:122-128,156), so a negative
+// cosine scores -1, not below it
 __device__ __forceinline__ float ref_score(int64_t dot, int64_t nq, int64_t np) {
     if (nq == 0 || np == 0) return -1.0f;
-    return (float)(2.0 * ((double)dot / (sqrt((double)nq) * sqrt((double)np)) - 0.5));
+    return fmaxf(-1.0f, (float)(2.0 * ((double)dot / (sqrt((double)nq) * sqrt((double)np)) - 0.5)));
 }
 
 // candidate list that follows category c (c < 0: the root)
@@ -299,46 +303,61 @@ desc_score(TreeDev t, const int16_t* __restrict__ q, int64_t qld, const int64_t*
                 lo[g] = t.lo + (int64_t)c * t.ld;
                 hi[g] = t.hi + (int64_t)c * t.ld;
             }
+            // A child with every |p| < 2^15 has a zero hi limb row (p = lo),
+            // which is then neither loaded nor multiplied: by its row maximum
+            // when the tree carries them, else when its squared norm is below
+            // 2^30 (which bounds every |p| below 2^15).
+            bool lo_only = true;
+#pragma unroll
+            for (int g = 0; g < GW; ++g)
+                if (g < G) {
+                    const int c = list[g0 + g];
+                    lo_only = lo_only && (t.row_maxabs ? t.row_maxabs[c] < 32768
+                                                       : t.sqnorm[c] < (int64_t(1) << 30));
+                }
             int64_t acc[kDescP][GW];
-            int32_t pl[kDescP][GW], ph[kDescP][GW];
 #pragma unroll
             for (int p = 0; p < kDescP; ++p)
 #pragma unroll
-                for (int g = 0; g < GW; ++g) { acc[p][g] = 0; pl[p][g] = 0; ph[p][g] = 0; }
-            for (int64_t k = 8 * lane; k < K; k += kDescStep) {
-                int4 qv[kDescP];
+                for (int g = 0; g < GW; ++g) acc[p][g] = 0;
+            auto pass = [&](auto HI) {
+                constexpr bool kHi = decltype(HI)::value;
+                for (int64_t k = 8 * lane; k < K; k += kDescStep) {
+                    int4 qv[kDescP];
 #pragma unroll
-                for (int p = 0; p < kDescP; ++p)
-                    qv[p] = ((mine >> p) & 1) ? *reinterpret_cast<const int4*>(qrow[p] + k)
-                                               : make_int4(0, 0, 0, 0);
+                    for (int p = 0; p < kDescP; ++p)
+                        qv[p] = ((mine >> p) & 1) ? *reinterpret_cast<const int4*>(qrow[p] + k)
+                                                   : make_int4(0, 0, 0, 0);
 #pragma unroll
-                for (int g = 0; g < GW; ++g) {
-                    if (g >= G) continue;  // uniform
-                    const int4 lv = *reinterpret_cast<const int4*>(lo[g] + k);
-                    const int4 hv = *reinterpret_cast<const int4*>(hi[g] + k);
-                    const int32_t l4[4] = {lv.x, lv.y, lv.z, lv.w};
-                    const int32_t h4[4] = {hv.x, hv.y, hv.z, hv.w};
+                    for (int g = 0; g < GW; ++g) {
+                        if (g >= G) continue;  // uniform
+                        const int4 lv = *reinterpret_cast<const int4*>(lo[g] + k);
+                        const int4 hv = kHi ? *reinterpret_cast<const int4*>(hi[g] + k)
+                                            : make_int4(0, 0, 0, 0);
+                        const int32_t l4[4] = {lv.x, lv.y, lv.z, lv.w};
+                        const int32_t h4[4] = {hv.x, hv.y, hv.z, hv.w};
 #pragma unroll
-                    for (int p = 0; p < kDescP; ++p) {
-                        const int32_t q4[4] = {qv[p].x, qv[p].y, qv[p].z, qv[p].w};
+                        for (int p = 0; p < kDescP; ++p) {
+                            const int32_t q4[4] = {qv[p].x, qv[p].y, qv[p].z, qv[p].w};
+                            int32_t pl = 0, ph = 0;
 #pragma unroll
-                        for (int d = 0; d < 4; ++d) {
-                            pl[p][g] = sdot2(q4[d], l4[d], pl[p][g]);
-                            ph[p][g] = sdot2(q4[d], h4[d], ph[p][g]);
-                            if (E == 2) {
-                                acc[p][g] += (int64_t)pl[p][g] + ((int64_t)ph[p][g] << 16);
-                                pl[p][g] = 0;
-                                ph[p][g] = 0;
+                            for (int d = 0; d < 4; ++d) {
+                                pl = sdot2(q4[d], l4[d], pl);
+                                if (kHi) ph = sdot2(q4[d], h4[d], ph);
+                                if (E == 2) {
+                                    acc[p][g] += kHi ? (int64_t)pl + ((int64_t)ph << 16) : (int64_t)pl;
+                                    pl = 0;
+                                    ph = 0;
+                                }
                             }
-                        }
-                        if (E == 8) {
-                            acc[p][g] += (int64_t)pl[p][g] + ((int64_t)ph[p][g] << 16);
-                            pl[p][g] = 0;
-                            ph[p][g] = 0;
+                            if (E == 8)
+                                acc[p][g] += kHi ? (int64_t)pl + ((int64_t)ph << 16) : (int64_t)pl;
                         }
                     }
                 }
-            }
+            };
+            if (lo_only) pass(std::false_type{});
+            else pass(std::true_type{});
 #pragma unroll
             for (int p = 0; p < kDescP; ++p)
 #pragma unroll
@@ -601,12 +620,15 @@ __global__ void tree_sum_kernel(int32_t* __restrict__ proto, int64_t ld,
 // int16 (p >= 2^31 - 32768) reports 65536 as its max |limb|
 __global__ void tree_limbs_kernel(const int32_t* __restrict__ proto, int64_t ld,
                                   int16_t* __restrict__ lo, int16_t* __restrict__ hi,
-                                  int64_t* __restrict__ sqnorm, int32_t* __restrict__ maxabs) {
+                                  int64_t* __restrict__ sqnorm, int32_t* __restrict__ maxabs,
+                                  int32_t* __restrict__ maxabs_p) {
     __shared__ int64_t s_n[kDescThreads];
     __shared__ int32_t s_m[kDescThreads];
+    __shared__ uint32_t s_p[kDescThreads];
     const int i = blockIdx.x;
     int64_t nrm = 0;
     int32_t mx = 0;
+    uint32_t mp = 0;  // max |p| (2^31 for INT32_MIN)
     for (int64_t k = threadIdx.x; k < ld; k += blockDim.x) {
         const int32_t p = proto[(int64_t)i * ld + k];
         const int32_t l = (int32_t)(int16_t)(p & 0xFFFF);
@@ -617,20 +639,24 @@ __global__ void tree_limbs_kernel(const int32_t* __restrict__ proto, int64_t ld,
         const int32_t al = l < 0 ? -l : l;
         const int32_t ah = h > 32767 ? 65536 : (int32_t)(h < 0 ? -h : h);
         mx = max(mx, max(al, ah));
+        mp = max(mp, p < 0 ? 0u - (uint32_t)p : (uint32_t)p);
     }
     s_n[threadIdx.x] = nrm;
     s_m[threadIdx.x] = mx;
+    s_p[threadIdx.x] = mp;
     __syncthreads();
     for (int o = kDescThreads / 2; o > 0; o >>= 1) {
         if ((int)threadIdx.x < o) {
             s_n[threadIdx.x] += s_n[threadIdx.x + o];
             s_m[threadIdx.x] = max(s_m[threadIdx.x], s_m[threadIdx.x + o]);
+            s_p[threadIdx.x] = max(s_p[threadIdx.x], s_p[threadIdx.x + o]);
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
         if (sqnorm) sqnorm[i] = s_n[0];
         if (maxabs) maxabs[i] = s_m[0];
+        if (maxabs_p) maxabs_p[i] = (int32_t)min(s_p[0], 0x7FFFFFFFu);
     }
 }
 
@@ -719,7 +745,7 @@ extern "C" int hvk_descend(hvk_ctx* ctx, const hvk_tree* tree, const int16_t* d_
                            dev, st))
         return e;
     const TreeDev t{tree->d_lo, tree->d_hi, tree->d_sqnorm, tld, n_cat,
-                    dev[0], dev[1], dev[2], tree->n_roots};
+                    dev[0], dev[1], dev[2], tree->n_roots, tree->d_row_maxabs};
 
     // workspace: two beam states, scores, grouped pairs, buckets, sort rows
     const size_t beam_bytes = align_up(n * 4) + align_up(n * B * 4) + 2 * align_up(n * B * L * 4);
@@ -882,7 +908,8 @@ extern "C" int hvk_tree_sum(hvk_ctx* ctx, int32_t* d_proto, int64_t ld, int n_ca
 
 extern "C" int hvk_tree_limbs(hvk_ctx* ctx, const int32_t* d_proto, int n, int64_t ld,
                               int16_t* d_lo, int16_t* d_hi, int64_t* d_sqnorm,
-                              int32_t* d_maxabs_limb, hvk_stream stream) {
+                              int32_t* d_maxabs_limb, int32_t* d_maxabs_proto,
+                              hvk_stream stream) {
     if (!ctx || n < 0) return HVK_ERR_ARG;
     if (n == 0) return HVK_OK;
     if (!d_proto || !d_lo || !d_hi) return HVK_ERR_ARG;
@@ -890,7 +917,7 @@ extern "C" int hvk_tree_limbs(hvk_ctx* ctx, const int32_t* d_proto, int n, int64
     if (ld < ctx->dim) return HVK_ERR_ARG;
     hipStream_t st = (hipStream_t)stream;
     hipLaunchKernelGGL(tree_limbs_kernel, dim3(n), dim3(kDescThreads), 0, st, d_proto, ld, d_lo,
-                       d_hi, d_sqnorm, d_maxabs_limb);
+                       d_hi, d_sqnorm, d_maxabs_limb, d_maxabs_proto);
     HVK_HIP(hipGetLastError());
     return HVK_OK;
 }

@@ -71,7 +71,8 @@ class Tree(ctypes.Structure):
                 ("d_hi", ctypes.c_void_p), ("d_sqnorm", ctypes.c_void_p),
                 ("max_abs_limb", ctypes.c_int32), ("child_off", ctypes.c_void_p),
                 ("child_idx", ctypes.c_void_p), ("roots", ctypes.c_void_p),
-                ("n_roots", ctypes.c_int), ("max_abs_proto", ctypes.c_int64)]
+                ("n_roots", ctypes.c_int), ("max_abs_proto", ctypes.c_int64),
+                ("d_row_maxabs", ctypes.c_void_p)]
 
 
 class GenSpec(ctypes.Structure):
@@ -136,7 +137,7 @@ def lib():
                               ctypes.c_float, i32, vp, vp, vp, vp, vp]
     L.hvk_tree_place.argtypes = [vp, vp, i64, i32, vp, vp, i64, i32, vp]
     L.hvk_tree_sum.argtypes = [vp, vp, i64, i32, vp, vp, vp, i32, vp, vp]
-    L.hvk_tree_limbs.argtypes = [vp, vp, i32, i64, vp, vp, vp, vp, vp]
+    L.hvk_tree_limbs.argtypes = [vp, vp, i32, i64, vp, vp, vp, vp, vp, vp]
     L.hvk_db_write.argtypes = [ctypes.c_char_p, i32, i32, i64, i32, ctypes.c_uint32,
                                ctypes.POINTER(EncodeOut), vp, i64]
     L.hvk_db_open.argtypes = [ctypes.c_char_p, ctypes.POINTER(vp)]
@@ -529,20 +530,23 @@ def make_taxonomy(sizes=(40, 200, 800, 3000, 12000, 60000), seq_len=1500,
 class TreeProto:
     """Device prototypes of a Taxonomy (int32 rows and their limbs)."""
 
-    def __init__(self, ctx, tax: Taxonomy, proto, lo, hi, sqnorm, maxabs):
+    def __init__(self, ctx, tax: Taxonomy, proto, lo, hi, sqnorm, maxabs, row_maxabs):
         self.tax, self.proto, self.lo, self.hi, self.sqnorm = tax, proto, lo, hi, sqnorm
+        self.row_maxabs = row_maxabs
         self.max_abs_limb = int(maxabs.max().item()) if maxabs.numel() else 0
         roots = _torch().from_numpy(tax.roots.astype(np.int64)).to(proto.device)
         self.max_abs_proto = (int(proto[roots].abs().max().item()) if len(tax.roots) else 0)
         self.ld = ctx.ld
         self.use_dense = True  # tests switch the level-0 matrix-core path off
+        self.use_row_maxabs = True  # tests switch the per-row maxima off
 
     def struct(self) -> Tree:
         t = self.tax
         return Tree(t.n_cat, self.ld, self.lo.data_ptr(), self.hi.data_ptr(),
                     self.sqnorm.data_ptr(), self.max_abs_limb, t.child_off.ctypes.data,
                     t.child_idx.ctypes.data, t.roots.ctypes.data, len(t.roots),
-                    self.max_abs_proto if self.use_dense else 0)
+                    self.max_abs_proto if self.use_dense else 0,
+                    self.row_maxabs.data_ptr() if self.use_row_maxabs else None)
 
 
 def build_prototypes(ctx: Context, tax: Taxonomy, leaf_counts, leaf_cat,
@@ -573,9 +577,10 @@ def build_prototypes(ctx: Context, tax: Taxonomy, leaf_counts, leaf_cat,
     hi = torch.empty_like(lo)
     sq = torch.empty(n_cat, dtype=torch.int64, device="cuda")
     mx = torch.empty(n_cat, dtype=torch.int32, device="cuda")
+    mp = torch.empty(n_cat, dtype=torch.int32, device="cuda")
     check(L.hvk_tree_limbs(ctx.h, _ptr(proto), n_cat, ld, _ptr(lo), _ptr(hi), _ptr(sq), _ptr(mx),
-                           _stream()), "hvk_tree_limbs")
-    return TreeProto(ctx, tax, proto, lo, hi, sq, mx)
+                           _ptr(mp), _stream()), "hvk_tree_limbs")
+    return TreeProto(ctx, tax, proto, lo, hi, sq, mx, mp)
 
 
 class Descent:

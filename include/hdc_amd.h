@@ -279,6 +279,9 @@ typedef struct {
     int64_t max_abs_proto;       /* max |p| over the roots' rows, 0 = unknown; when below
                                     2^30 and n_roots <= 64 the level-0 scores run as an
                                     int8-limb GEMM on the matrix cores */
+    const int32_t* d_row_maxabs; /* [n_cat] max |p| of each row, nullable: a row below
+                                    2^15 has a zero hi limb, which is then skipped (without
+                                    it, rows whose d_sqnorm is below 2^30 are) */
 } hvk_tree;
 
 /* Outputs per read (device, caller-allocated): path[levels] category ids and
@@ -298,7 +301,8 @@ int hvk_descend(hvk_ctx* ctx, const hvk_tree* tree, const int16_t* d_q_counts,
  * of its candidate list child_idx[child_off[p] .. child_off[p+1]) (call it
  * rank by rank from the leaves up); a sum outside int32 sets *d_overflow
  * (nullable) to 1.  hvk_tree_limbs splits int32 rows into the lo/hi limbs
- * and exact squared norms hvk_descend reads, and each row's max |limb|. */
+ * and exact squared norms hvk_descend reads, each row's max |limb|, and
+ * (d_maxabs_proto, nullable) each row's max |p|, the tree's d_row_maxabs. */
 int hvk_tree_place(hvk_ctx* ctx, const int16_t* d_counts, int64_t counts_ld, int n,
                    const int32_t* h_cat, int32_t* d_proto, int64_t ld, int n_cat,
                    hvk_stream stream);
@@ -306,7 +310,8 @@ int hvk_tree_sum(hvk_ctx* ctx, int32_t* d_proto, int64_t ld, int n_cat, const in
                  const int32_t* h_child_idx, const int32_t* h_parents, int n_parents,
                  int32_t* d_overflow, hvk_stream stream);
 int hvk_tree_limbs(hvk_ctx* ctx, const int32_t* d_proto, int n, int64_t ld, int16_t* d_lo,
-                   int16_t* d_hi, int64_t* d_sqnorm, int32_t* d_maxabs_limb, hvk_stream stream);
+                   int16_t* d_hi, int64_t* d_sqnorm, int32_t* d_maxabs_limb,
+                   int32_t* d_maxabs_proto, hvk_stream stream);
 
 /* ---- prototype database file ---------------------------------------------
  * A versioned binary file of encoded reference prototypes, so references are

@@ -107,6 +107,7 @@ def test_prototypes_match_oracle(synth):
     assert np.array_equal(65536 * hi[:, :D] + lo[:, :D], proto.astype(np.int64))
     sq = (proto.astype(np.int64) ** 2).sum(axis=1)
     assert np.array_equal(tree.sqnorm.cpu().numpy(), sq)
+    assert np.array_equal(tree.row_maxabs.cpu().numpy(), np.abs(proto.astype(np.int64)).max(axis=1))
 
 
 def _run_both(synth, beam, thr, levels=6, q_maxabs=None):
@@ -177,6 +178,52 @@ def test_descend_wide_candidate_lists_match_oracle():
         ref = orc.descend(proto, tax.child_off, tax.child_idx, tax.roots, qc, beam, -1.0, 6)
         _assert_same((res.path.cpu().numpy(), res.score.cpu().numpy(),
                       res.length.cpu().numpy(), res.exit_level.cpu().numpy()), ref)
+    ctx.close()
+
+
+def test_descend_skips_zero_hi_limbs_exactly():
+    """Rows with |p| >= 2^15 (a parent of 300 equal leaves) need the hi limb;
+    leaf rows do not and skip it.  A candidate list mixing both, and the
+    two tests of a zero hi row (per-row maxima, squared norms), give the
+    oracle's paths and scores."""
+    import hdc_amd as H
+    import torch
+    D, k = 10240, 12
+    rng = np.random.default_rng(21)
+    s1 = "".join(rng.choice(list("ACGT"), 1500))
+    s2 = "".join(rng.choice(list("ACGT"), 1500))
+    # 0: root; 1: A (300 leaves equal to s1); 2: B (one leaf s2); 3..302: A's
+    # leaves; 303: B's leaf.  Leaves list themselves.
+    n_leaf_a = 300
+    n_cat = 3 + n_leaf_a + 1
+    kids = [[1, 2], list(range(3, 3 + n_leaf_a)), [3 + n_leaf_a]] + \
+        [[c] for c in range(3, n_cat)]
+    off = np.concatenate([[0], np.cumsum([len(x) for x in kids])]).astype(np.int32)
+    idx = np.concatenate(kids).astype(np.int32)
+    ranks = [np.array([0]), np.array([1, 2]), np.arange(3, n_cat)]
+    tax = H.Taxonomy(off, idx, [0], ranks)
+    leaf_seqs = [s1] * n_leaf_a + [s2]
+    reads = [s1[i:i + 400] for i in range(0, 1000, 100)] + \
+        [s2[i:i + 700] for i in range(0, 800, 200)] + ["".join(rng.choice(list("ACGT"), 500))]
+    ctx = H.Context(D, k)
+    leaf_enc = H.encode(ctx, _dev_seqs(H, leaf_seqs), counts=True, limbs=False, hi=False)
+    tree = H.build_prototypes(ctx, tax, leaf_enc.counts, ranks[-1])
+    q = H.encode(ctx, _dev_seqs(H, reads), counts=True, limbs=False, hi=False)
+    torch.cuda.synchronize()
+    proto = _oracle_protos(D, k, tax, leaf_seqs)
+    row_max = np.abs(proto.astype(np.int64)).max(axis=1)
+    assert np.array_equal(tree.row_maxabs.cpu().numpy(), row_max)
+    assert row_max[1] >= 32768 and row_max[2] < 32768 and row_max[3:].max() < 32768
+    qc = q.counts.cpu().numpy()[:, :D]
+    for use_rows in (True, False):
+        tree.use_row_maxabs = use_rows
+        for beam in (1, 3):
+            print(f"descend hi-limb tree rows={use_rows} beam={beam}", flush=True)
+            res = H.descend(ctx, tree, q, beam=beam, threshold=-1.0, levels=3)
+            torch.cuda.synchronize()
+            ref = orc.descend(proto, off, idx, [0], qc, beam, -1.0, 3)
+            _assert_same((res.path.cpu().numpy(), res.score.cpu().numpy(),
+                          res.length.cpu().numpy(), res.exit_level.cpu().numpy()), ref)
     ctx.close()
 
 
