@@ -379,11 +379,12 @@ desc_score(TreeDev t, const int16_t* __restrict__ q, int64_t qld, const int64_t*
 // the same candidate list (the roots), so the scores are a dense product
 // reads x roots x D.  Operands are split into signed base-256 digits: a read
 // count q = q0 + 256 q1 (|q| <= 32639), a prototype entry
-// p = d0 + 256 d1 + 2^16 d2 + 2^24 d3 (|p| < 2^30), all in [-128, 127].
-// v_mfma_i32_32x32x32_i8 multiplies digit pairs; the 8 products fall into 5
-// weights 2^(8w) (w = a + b), each accumulated in int32 -- at most 2 products
-// x 10240 x 128 x 128 < 2^31 -- and combined in int64 at the end: the exact
-// dot.  A wave owns 32 reads; the workgroup (4 waves) stages each 32-element
+// p = d0 + 256 d1 + ... over ND digits, all in [-128, 127]: ND = 2 covers
+// |p| <= 32639, 3 covers |p| <= 8355711, 4 every |p| < 2^30 (the host picks
+// the fewest for the roots' largest |p|).  v_mfma_i32_32x32x32_i8 multiplies
+// digit pairs; the 2 ND products fall into ND + 1 weights 2^(8w) (w = a + b),
+// each accumulated in int32 -- at most 2 products x 10240 x 128 x 128 < 2^31
+// -- and combined in int64 at the end: the exact dot.  A wave owns 32 reads; the workgroup (4 waves) stages each 32-element
 // K step of the roots' digits in LDS once.  Lane l holds row / column l % 32
 // and K bytes [16 (l / 32), +16) of both operands, as the MFMA pairs them.
 typedef int v4i_t __attribute__((ext_vector_type(4)));
@@ -396,12 +397,14 @@ __device__ __forceinline__ uint32_t pack4(int a, int b, int c, int d) {
            ((uint32_t)(d & 0xFF) << 24);
 }
 
-// (Two workgroups per CU: the 160 accumulator registers and the rest fit in
-// 256, and the second wave per SIMD hides the read-row loads.)
+// (Two workgroups per CU: the accumulator registers -- 160 at ND = 4 -- and
+// the rest fit in 256, and the second wave per SIMD hides the read-row loads.)
+template <int ND>
 __global__ void __launch_bounds__(256, 2)
 desc_dense(TreeDev t, const int16_t* __restrict__ q, int64_t qld, const int64_t* __restrict__ qn,
            int n, int B, int64_t K, int Fmax, float* __restrict__ sc) {
-    __shared__ __attribute__((aligned(16))) int8_t bl[4][kDenseN][kDenseRow];
+    static_assert(ND >= 2 && ND <= 4, "prototype digits");
+    __shared__ __attribute__((aligned(16))) int8_t bl[ND][kDenseN][kDenseRow];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int r0 = (blockIdx.x * 4 + wave) * 32;
     const int i = lane & 31, kg = lane >> 5;
@@ -411,39 +414,40 @@ desc_dense(TreeDev t, const int16_t* __restrict__ q, int64_t qld, const int64_t*
     const int catB = bc < t.n_roots ? t.roots[bc] : -1;
     const int16_t* __restrict__ blo = t.lo + (int64_t)(catB < 0 ? 0 : catB) * t.ld;
     const int16_t* __restrict__ bhi = t.hi + (int64_t)(catB < 0 ? 0 : catB) * t.ld;
-    v16i_t acc[5][2];
+    v16i_t acc[ND + 1][2];
 #pragma unroll
-    for (int w = 0; w < 5; ++w)
+    for (int w = 0; w < ND + 1; ++w)
 #pragma unroll
         for (int nb = 0; nb < 2; ++nb) acc[w][nb] = v16i_t{};
     for (int64_t k0 = 0; k0 < K; k0 += 32) {
         __syncthreads();  // the previous step's digits are consumed
         {
-            int d[4][8];
+            int d[ND][8];
             if (catB >= 0) {
                 const int4 lv = *reinterpret_cast<const int4*>(blo + k0 + bk);
-                const int4 hv = *reinterpret_cast<const int4*>(bhi + k0 + bk);
+                const int4 hv = ND > 2 ? *reinterpret_cast<const int4*>(bhi + k0 + bk)
+                                       : make_int4(0, 0, 0, 0);  // |p| < 2^15: hi is zero
                 const int lw[4] = {lv.x, lv.y, lv.z, lv.w}, hw[4] = {hv.x, hv.y, hv.z, hv.w};
 #pragma unroll
                 for (int e = 0; e < 8; ++e) {
                     const int lo = (int)(int16_t)(lw[e >> 1] >> (16 * (e & 1)));
                     const int hi = (int)(int16_t)(hw[e >> 1] >> (16 * (e & 1)));
                     int p = lo + hi * 65536;
-                    d[0][e] = (int)(int8_t)p;
-                    p = (p - d[0][e]) >> 8;
-                    d[1][e] = (int)(int8_t)p;
-                    p = (p - d[1][e]) >> 8;
-                    d[2][e] = (int)(int8_t)p;
-                    d[3][e] = (p - d[2][e]) >> 8;
+#pragma unroll
+                    for (int g = 0; g + 1 < ND; ++g) {
+                        d[g][e] = (int)(int8_t)p;
+                        p = (p - d[g][e]) >> 8;
+                    }
+                    d[ND - 1][e] = p;
                 }
             } else {
 #pragma unroll
-                for (int g = 0; g < 4; ++g)
+                for (int g = 0; g < ND; ++g)
 #pragma unroll
                     for (int e = 0; e < 8; ++e) d[g][e] = 0;
             }
 #pragma unroll
-            for (int g = 0; g < 4; ++g)
+            for (int g = 0; g < ND; ++g)
                 *reinterpret_cast<uint2*>(&bl[g][bc][bk]) =
                     make_uint2(pack4(d[g][0], d[g][1], d[g][2], d[g][3]),
                                pack4(d[g][4], d[g][5], d[g][6], d[g][7]));
@@ -469,7 +473,7 @@ desc_dense(TreeDev t, const int16_t* __restrict__ q, int64_t qld, const int64_t*
 #pragma unroll
         for (int nb = 0; nb < 2; ++nb)
 #pragma unroll
-            for (int g = 0; g < 4; ++g) {
+            for (int g = 0; g < ND; ++g) {
                 const v4i_t b = *reinterpret_cast<const v4i_t*>(&bl[g][nb * 32 + i][16 * kg]);
 #pragma unroll
                 for (int qa_i = 0; qa_i < 2; ++qa_i)
@@ -487,7 +491,7 @@ desc_dense(TreeDev t, const int16_t* __restrict__ q, int64_t qld, const int64_t*
             if (r < n && col < t.n_roots) {
                 int64_t dot = 0;
 #pragma unroll
-                for (int w = 0; w < 5; ++w) dot += (int64_t)acc[w][nb][e] << (8 * w);
+                for (int w = 0; w < ND + 1; ++w) dot += (int64_t)acc[w][nb][e] << (8 * w);
                 sc[(int64_t)r * B * Fmax + col] = ref_score(dot, qn[r], t.sqnorm[t.roots[col]]);
             }
         }
@@ -798,7 +802,10 @@ extern "C" int hvk_descend(hvk_ctx* ctx, const hvk_tree* tree, const int16_t* d_
     int cur = 0;
     for (int lvl = 0; lvl < L; ++lvl) {
         if (lvl == 0 && dense) {  // every read has one entry, the root
-            hipLaunchKernelGGL(desc_dense, dim3((n_q + 127) / 128), dim3(256), 0, st, t, d_q_counts,
+            auto dense_k = tree->max_abs_proto <= 32639     ? desc_dense<2>
+                           : tree->max_abs_proto <= 8355711 ? desc_dense<3>
+                                                            : desc_dense<4>;
+            hipLaunchKernelGGL(dense_k, dim3((n_q + 127) / 128), dim3(256), 0, st, t, d_q_counts,
                                q_ld, d_q_sqnorm, n_q, B, K, Fmax, sc);
         } else {
             hipLaunchKernelGGL(desc_fill, dim3((n_keys + kDescThreads - 1) / kDescThreads),

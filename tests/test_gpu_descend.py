@@ -307,18 +307,24 @@ def test_descend_rejects_bad_trees():
 
 def test_dense_level0_equals_vector_path(synth):
     """Level 0 on the matrix cores (int8 digit GEMM) and on the vector path give
-    identical descents."""
+    identical descents, with the prototypes split into 2, 3 and 4 digits (a
+    larger claimed max |p| than the true one only adds digits)."""
     import hdc_amd as H
     import torch
     ctx, tax, tree, q, proto, qc, truth = synth
+    true_max = tree.max_abs_proto
+    assert 0 < true_max <= 32639  # two digits
     outs = []
-    for dense in (True, False):
+    for dense, claim in ((False, true_max), (True, true_max), (True, 40000), (True, 9_000_000)):
         tree.use_dense = dense
-        print(f"descend synthetic dense_level0={dense}", flush=True)
+        tree.max_abs_proto = claim
+        print(f"descend synthetic dense_level0={dense} max_abs_proto={claim}", flush=True)
         r = H.descend(ctx, tree, q, beam=3, threshold=-1.0, levels=6)
         torch.cuda.synchronize()
         outs.append((r.path.cpu().numpy(), r.score.cpu().numpy().view(np.int32),
                      r.length.cpu().numpy(), r.exit_level.cpu().numpy()))
     tree.use_dense = True
-    for a, b in zip(*outs):
-        assert np.array_equal(a, b)
+    tree.max_abs_proto = true_max
+    for other in outs[1:]:
+        for a, b in zip(outs[0], other):
+            assert np.array_equal(a, b)
