@@ -109,7 +109,7 @@ def cpu_rows(w, n_take, db_host, dbn, n_refs, dim, kmer, topk, gpu_idx, cpu_seco
                       f"wall, {n_all} threads); 1-thread row: first {rows['cpu_1']['reads']} reads",
             "flags": flags, "nproc": nproc, "cores_available": avail, "cpu_model": model,
             "cpu_1": rows["cpu_1"], "cpu_all": a,
-            "top5_equal_gpu": bool(a["topk_equal_gpu"] and rows["cpu_1"]["topk_equal_gpu"])}
+            "topk_equal_gpu": bool(a["topk_equal_gpu"] and rows["cpu_1"]["topk_equal_gpu"])}
 
 
 class PowerSampler:
