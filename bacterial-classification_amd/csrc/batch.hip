@@ -57,6 +57,29 @@ struct Graph {
     hipGraphExec_t exec;
 };
 
+// A descent chunk's graph: besides the chunk and the batch's buffers, every
+// scalar the launches take from the plan (the candidate lists' contents are
+// uploaded by each call into the same buffer and read at replay).
+struct DescKey {
+    int64_t c0;
+    int n, width;
+    int E, B, L, dense, n_cat, n_roots, Fmax, maxN, max_span;
+    uint32_t thr_bits;
+    int64_t max_abs_proto, tld;
+    const void* tree[7];  // lo, hi, sqnorm, row maxima, child_off, child_idx, roots
+    const void* in;
+    const void* order;
+    const void* rows;
+    const void* outs;
+    const void* ws;  // the descent workspace
+    bool operator==(const DescKey& o) const { return std::memcmp(this, &o, sizeof(*this)) == 0; }
+};
+
+struct DescGraph {
+    DescKey key;
+    hipGraphExec_t exec;
+};
+
 }  // namespace
 
 struct hvk_batch {
@@ -75,6 +98,7 @@ struct hvk_batch {
     std::vector<int64_t> starts;
     std::vector<int64_t> offsets;  // the offsets the order was computed from
     std::vector<Graph> graphs;
+    std::vector<DescGraph> dgraphs;
 };
 
 namespace hvk {
@@ -83,6 +107,8 @@ namespace {
 void drop_graphs(hvk_batch* b) {
     for (auto& g : b->graphs) (void)hipGraphExecDestroy(g.exec);
     b->graphs.clear();
+    for (auto& g : b->dgraphs) (void)hipGraphExecDestroy(g.exec);
+    b->dgraphs.clear();
 }
 
 // out[perm[i]][j] = chunk[i][j]; block 0 adds the chunk's re-scored count
@@ -339,6 +365,191 @@ int classify(hvk_batch* b, const uint32_t* caller_packed, const uint32_t* caller
     return HVK_OK;
 }
 
+// ---- the descent in the batching form -----------------------------------
+// Per length-ordered chunk: encode (int16 counts and norms) and the
+// descent's launches, captured once and replayed; the chunk's paths go to
+// the reads' own rows outside the graph.
+
+// out[perm[i]] = chunk[i] for the path rows (L entries) and per-read values
+__global__ void batch_desc_scatter(const int32_t* __restrict__ perm, int n, int L,
+                                   const int32_t* __restrict__ cp, const float* __restrict__ cs,
+                                   const int32_t* __restrict__ cl, const int32_t* __restrict__ ce,
+                                   int32_t* __restrict__ op, float* __restrict__ os,
+                                   int32_t* __restrict__ ol, int32_t* __restrict__ oe) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n * L) return;
+    const int i = (int)(t / L), j = (int)(t - (int64_t)i * L);
+    const int64_t r = perm[i];
+    op[r * L + j] = cp[t];
+    os[r * L + j] = cs[t];
+    if (j == 0) {
+        ol[r] = cl[i];
+        oe[r] = ce[i];
+    }
+}
+
+struct DescBufs {
+    int16_t* counts;
+    int64_t* sqnorm;
+    int32_t* maxabs;
+    int32_t* nkmers;
+    int32_t* path;
+    float* pscore;
+    int32_t* len;
+    int32_t* exit;
+};
+
+int desc_bufs(hvk_batch* b, int L, DescBufs& cb) {
+    const int64_t ld = b->ctx->ld;
+    const size_t m = (size_t)b->chunk;
+    if (int e = b->ws_rows.ensure(al(m * ld * 2) + al(m * 8) + 2 * al(m * 4))) return e;
+    if (int e = b->ws_out.ensure(2 * al(m * L * 4) + 2 * al(m * 4))) return e;
+    char* p = (char*)b->ws_rows.ptr;
+    auto take = [&](size_t n) { char* q = p; p += al(n); return q; };
+    cb.counts = (int16_t*)take(m * ld * 2);
+    cb.sqnorm = (int64_t*)take(m * 8);
+    cb.maxabs = (int32_t*)take(m * 4);
+    cb.nkmers = (int32_t*)take(m * 4);
+    p = (char*)b->ws_out.ptr;
+    cb.path = (int32_t*)take(m * L * 4);
+    cb.pscore = (float*)take(m * L * 4);
+    cb.len = (int32_t*)take(m * 4);
+    cb.exit = (int32_t*)take(m * 4);
+    return HVK_OK;
+}
+
+int run_desc_chunk(hvk_batch* b, const uint32_t* packed, const uint32_t* invalid, int64_t c0,
+                   int n, int max_len, const DescPlan& pl, float threshold, const DescBufs& cb,
+                   const int64_t* starts, const int32_t* lens) {
+    hvk_ctx* ctx = b->ctx;
+    EncodeOut eo{};
+    eo.counts = cb.counts;
+    eo.ld = ctx->ld;
+    eo.sqnorm = cb.sqnorm;
+    eo.maxabs = cb.maxabs;
+    eo.nkmers = cb.nkmers;
+    if (int e = launch_encode(ctx, packed, invalid, nullptr, starts + c0, 0, n, max_len, eo, b->s,
+                              lens + c0))
+        return e;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    return desc_launch(ctx, pl, cb.counts, cb.sqnorm, n, ctx->ld, threshold, cb.path, cb.pscore,
+                       cb.len, cb.exit, b->s);
+}
+
+int descend(hvk_batch* b, const hvk_tree* tree, const uint32_t* caller_packed,
+            const uint32_t* caller_invalid, const int64_t* h_offsets, int n, int beam,
+            float threshold, int levels, int32_t* op, float* os, int32_t* ol, int32_t* oe,
+            hipStream_t caller) {
+    HVK_HIP(hipEventRecord(b->ev_in, caller));
+    HVK_HIP(hipStreamWaitEvent(b->s, b->ev_in, 0));
+    int max_len = 0;
+    if (int e = order_reads(b, h_offsets, n, max_len)) return e;
+    // |count| <= the read's valid k-mers <= its length - k + 1: the bound of
+    // the longest read serves every chunk (one plan per call)
+    const int q_bound = std::max(0, max_len - b->ctx->k + 1);
+    DescPlan pl;
+    {
+        std::lock_guard<std::mutex> g(b->ctx->mu);
+        if (int e = desc_prepare(b->ctx, tree, q_bound, beam, levels, pl, b->s)) return e;
+    }
+    const size_t words = (size_t)hvk_pack_words(h_offsets[n]);
+    if (int e = b->ws_in.ensure(2 * al(words * 4))) return e;
+    uint32_t* packed = (uint32_t*)b->ws_in.ptr;
+    uint32_t* invalid = (uint32_t*)((char*)b->ws_in.ptr + al(words * 4));
+    HVK_HIP(hipMemcpyAsync(packed, caller_packed, words * 4, hipMemcpyDeviceToDevice, b->s));
+    HVK_HIP(hipMemcpyAsync(invalid, caller_invalid, words * 4, hipMemcpyDeviceToDevice, b->s));
+    const char* ordp = (const char*)b->ws_order.ptr;
+    const int32_t* perm = (const int32_t*)ordp;
+    const int32_t* lens = (const int32_t*)(ordp + al((size_t)n * 4));
+    const int64_t* starts = (const int64_t*)(ordp + 2 * al((size_t)n * 4));
+    DescBufs cb;
+    if (int e = desc_bufs(b, levels, cb)) return e;
+    auto scatter = [&](int64_t c0, int m) -> int {
+        const int64_t tot = (int64_t)m * levels;
+        hipLaunchKernelGGL(batch_desc_scatter, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                           b->s, perm + c0, m, levels, cb.path, cb.pscore, cb.len, cb.exit, op,
+                           os, ol, oe);
+        HVK_HIP(hipGetLastError());
+        return HVK_OK;
+    };
+    for (int64_t c0 = 0; c0 < n; c0 += b->chunk) {
+        const int m = (int)std::min<int64_t>(b->chunk, n - c0);
+        const int chunk_max = b->lens[(size_t)c0];  // longest first
+        auto make_key = [&]() {
+            DescKey key;
+            std::memset(&key, 0, sizeof(key));
+            key.c0 = c0;
+            key.n = m;
+            key.width = width_class(chunk_max, b->ctx->k);
+            key.E = pl.E;
+            key.B = pl.B;
+            key.L = pl.L;
+            key.dense = pl.dense;
+            key.n_cat = pl.n_cat;
+            key.n_roots = pl.n_roots;
+            key.Fmax = pl.Fmax;
+            key.maxN = pl.maxN;
+            key.max_span = pl.max_span;
+            std::memcpy(&key.thr_bits, &threshold, 4);
+            key.max_abs_proto = pl.max_abs_proto;
+            key.tld = pl.t.ld;
+            key.tree[0] = pl.t.lo;
+            key.tree[1] = pl.t.hi;
+            key.tree[2] = pl.t.sqnorm;
+            key.tree[3] = pl.t.row_maxabs;
+            key.tree[4] = pl.t.child_off;
+            key.tree[5] = pl.t.child_idx;
+            key.tree[6] = pl.t.roots;
+            key.in = b->ws_in.ptr;
+            key.order = b->ws_order.ptr;
+            key.rows = b->ws_rows.ptr;
+            key.outs = b->ws_out.ptr;
+            key.ws = b->ctx->ws_desc.ptr;
+            return key;
+        };
+        DescKey key = make_key();
+        auto it = std::find_if(b->dgraphs.begin(), b->dgraphs.end(),
+                               [&](const DescGraph& g) { return g.key == key; });
+        if (it != b->dgraphs.end()) {
+            HVK_HIP(hipGraphLaunch(it->exec, b->s));
+            std::rotate(it, it + 1, b->dgraphs.end());  // most recently used last
+            if (int e = scatter(c0, m)) return e;
+            continue;
+        }
+        // direct run (sizes the descent workspace), then capture the same launches
+        if (int e = run_desc_chunk(b, packed, invalid, c0, m, chunk_max, pl, threshold, cb, starts,
+                                   lens))
+            return e;
+        if (int e = scatter(c0, m)) return e;
+        key = make_key();  // the run may have grown (moved) the workspace
+        if (b->dgraphs.size() >= kMaxGraphs) {
+            HVK_HIP(hipStreamSynchronize(b->s));
+            while (b->dgraphs.size() >= kMaxGraphs) {
+                (void)hipGraphExecDestroy(b->dgraphs.front().exec);
+                b->dgraphs.erase(b->dgraphs.begin());
+            }
+        }
+        HVK_HIP(hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal));
+        const int ce = run_desc_chunk(b, packed, invalid, c0, m, chunk_max, pl, threshold, cb,
+                                      starts, lens);
+        hipGraph_t g = nullptr;
+        const hipError_t he = hipStreamEndCapture(b->s, &g);
+        if (ce) {
+            if (g) (void)hipGraphDestroy(g);
+            return ce;
+        }
+        if (he != hipSuccess) { set_last_error("hipStreamEndCapture", he); return HVK_ERR_HIP; }
+        hipGraphExec_t ex = nullptr;
+        const hipError_t ie = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) { set_last_error("hipGraphInstantiate", ie); return HVK_ERR_HIP; }
+        b->dgraphs.push_back({key, ex});
+    }
+    HVK_HIP(hipEventRecord(b->ev_out, b->s));
+    HVK_HIP(hipStreamWaitEvent(caller, b->ev_out, 0));
+    return HVK_OK;
+}
+
 int check_inputs(const hvk_batch* b, const uint32_t* packed, const uint32_t* invalid,
                  const int64_t* h_offsets, int n, const hvk_bundles* db, int k, const int32_t* oi,
                  const float* os) {
@@ -498,4 +709,18 @@ extern "C" int hvk_batch_autotune(hvk_batch* b, const uint32_t* d_packed,
     if (ms_per_mread) *ms_per_mread = best * 1e6f / (float)m;
     if (table_rows) *table_rows = std::min(rows, cap);
     return HVK_OK;
+}
+
+extern "C" int hvk_batch_descend(hvk_batch* b, const hvk_tree* tree, const uint32_t* d_packed,
+                                 const uint32_t* d_invalid, const int64_t* h_offsets, int n,
+                                 int beam, float threshold, int levels, int32_t* d_path,
+                                 float* d_path_score, int32_t* d_path_len, int32_t* d_exit_level,
+                                 hvk_stream stream) {
+    if (!b || !tree || n < 0) return HVK_ERR_ARG;
+    if (n == 0) return HVK_OK;
+    if (!d_packed || !d_invalid || !h_offsets || !d_path || !d_path_score || !d_path_len ||
+        !d_exit_level)
+        return HVK_ERR_ARG;
+    return descend(b, tree, d_packed, d_invalid, h_offsets, n, beam, threshold, levels, d_path,
+                   d_path_score, d_path_len, d_exit_level, (hipStream_t)stream);
 }

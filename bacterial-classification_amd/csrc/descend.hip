@@ -50,19 +50,6 @@ struct Beams {
     float* pscore;  // [n][B][L]
 };
 
-struct TreeDev {
-    const int16_t* lo;
-    const int16_t* hi;
-    const int64_t* sqnorm;
-    int64_t ld;
-    int n_cat;
-    const int32_t* child_off;  // [n_cat + 1]
-    const int32_t* child_idx;
-    const int32_t* roots;
-    int n_roots;
-    const int32_t* row_maxabs;  // [n_cat] max |p|, nullable
-};
-
 // src/HV.cpp:236-241 on the exact integer dot and norms, as a candidate's
 // score: identify starts every candidate at -1.0f and keeps the max of that
 // and its similarity (targets/identify.cpp:122-128,156), so a negative
@@ -693,24 +680,22 @@ int upload_i32(hvk_ctx* ctx, Workspace& ws, std::initializer_list<std::pair<cons
 
 using namespace hvk;
 
-extern "C" int hvk_descend(hvk_ctx* ctx, const hvk_tree* tree, const int16_t* d_q_counts,
-                           const int64_t* d_q_sqnorm, int n_q, int64_t q_ld, int q_maxabs,
-                           int beam, float threshold, int levels, int32_t* d_path,
-                           float* d_path_score, int32_t* d_path_len, int32_t* d_exit_level,
-                           hvk_stream stream) {
-    if (!ctx || !tree || n_q < 0 || beam < 1 || beam > kMaxBeam || levels < 1 ||
-        levels > kMaxLevels || q_maxabs < 0)
-        return HVK_ERR_ARG;
-    if (n_q == 0) return HVK_OK;
-    if (!d_q_counts || !d_q_sqnorm || !d_path || !d_path_score || !d_path_len || !d_exit_level)
+namespace hvk {
+
+// Checks the tree and the read bound, picks the partial-sum chunk E and the
+// level-0 form, and uploads the candidate lists to the context (the copies
+// come from the caller's host arrays and are waited for).  Not capturable.
+int desc_prepare(hvk_ctx* ctx, const hvk_tree* tree, int q_maxabs, int beam, int levels,
+                 DescPlan& pl, hipStream_t st) {
+    if (!ctx || !tree || beam < 1 || beam > kMaxBeam || levels < 1 || levels > kMaxLevels ||
+        q_maxabs < 0)
         return HVK_ERR_ARG;
     const int n_cat = tree->n_cat;
     if (n_cat < 0 || tree->n_roots < 0 || !tree->child_off || (tree->n_roots && !tree->roots))
         return HVK_ERR_ARG;
     if (n_cat > 0 && (!tree->d_lo || !tree->d_hi || !tree->d_sqnorm)) return HVK_ERR_ARG;
-    if (q_ld == 0) q_ld = ctx->ld;
     const int64_t tld = tree->ld == 0 ? ctx->ld : tree->ld;
-    if (q_ld < ctx->dim || tld < ctx->dim || q_ld % 128 || tld % 128) return HVK_ERR_ARG;
+    if (tld < ctx->dim || tld % 128) return HVK_ERR_ARG;
     // candidate lists: offsets monotone from 0, indices in range
     if (tree->child_off[0] != 0) return HVK_ERR_ARG;
     int max_span = 0;
@@ -729,18 +714,21 @@ extern "C" int hvk_descend(hvk_ctx* ctx, const hvk_tree* tree, const int16_t* d_
     if (tree->max_abs_limb < 0 || tree->max_abs_limb > 32768) return HVK_ERR_RANGE;
     const int64_t prod = (int64_t)q_maxabs * tree->max_abs_limb;
     const int64_t lim = int64_t(1) << 31;
-    int E;
-    if (8 * prod < lim) E = 8;
-    else if (2 * prod < lim) E = 2;
+    if (8 * prod < lim) pl.E = 8;
+    else if (2 * prod < lim) pl.E = 2;
     else return HVK_ERR_RANGE;
-
-    const int B = beam, L = levels;
-    const int Fmax = std::max(1, std::max(tree->n_roots, max_span));
-    const int maxN = std::max(1, std::max(tree->n_roots, B * max_span));
-    const int64_t n = n_q;
-    std::lock_guard<std::mutex> g(ctx->mu);
-    hipStream_t st = (hipStream_t)stream;
-
+    pl.B = beam;
+    pl.L = levels;
+    pl.max_span = max_span;
+    pl.Fmax = std::max(1, std::max(tree->n_roots, max_span));
+    pl.maxN = std::max(1, std::max(tree->n_roots, beam * max_span));
+    pl.n_cat = n_cat;
+    pl.n_roots = tree->n_roots;
+    pl.max_abs_proto = tree->max_abs_proto;
+    // level 0 on the matrix cores when its operands fit the digit split; each
+    // digit-weight class sums < 2 K 2^14 in int32
+    pl.dense = tree->n_roots > 0 && tree->n_roots <= kDenseN && tree->max_abs_proto > 0 &&
+               tree->max_abs_proto < (int64_t(1) << 30) && q_maxabs <= 32639 && ctx->ld < 65536;
     std::vector<const int32_t*> dev;
     if (int e = upload_i32(ctx, ctx->ws_desc_tree,
                            {{tree->child_off, (size_t)n_cat + 1},
@@ -748,9 +736,26 @@ extern "C" int hvk_descend(hvk_ctx* ctx, const hvk_tree* tree, const int16_t* d_
                             {tree->roots, (size_t)tree->n_roots}},
                            dev, st))
         return e;
-    const TreeDev t{tree->d_lo, tree->d_hi, tree->d_sqnorm, tld, n_cat,
-                    dev[0], dev[1], dev[2], tree->n_roots, tree->d_row_maxabs};
+    pl.t = TreeDev{tree->d_lo, tree->d_hi, tree->d_sqnorm, tld, n_cat,
+                   dev[0], dev[1], dev[2], tree->n_roots, tree->d_row_maxabs};
+    return HVK_OK;
+}
 
+// The launches of a descent on a prepared plan: the workspace (grown here
+// when it is too small) and six launches per level on st, with no host
+// synchronisation.  A stream capture of it records launches only when the
+// workspace already holds n_q reads.
+int desc_launch(hvk_ctx* ctx, const DescPlan& pl, const int16_t* d_q_counts,
+                const int64_t* d_q_sqnorm, int n_q, int64_t q_ld, float threshold,
+                int32_t* d_path, float* d_path_score, int32_t* d_path_len,
+                int32_t* d_exit_level, hipStream_t st) {
+    if (n_q == 0) return HVK_OK;
+    const TreeDev& t = pl.t;
+    const int n_cat = pl.n_cat;
+    const int B = pl.B, L = pl.L, E = pl.E;
+    const int Fmax = pl.Fmax, maxN = pl.maxN, max_span = pl.max_span;
+    const int64_t n = n_q;
+    const bool dense = pl.dense;
     // workspace: two beam states, scores, grouped pairs, buckets, sort rows
     const size_t beam_bytes = align_up(n * 4) + align_up(n * B * 4) + 2 * align_up(n * B * L * 4);
     const size_t sc_bytes = align_up((size_t)n * B * Fmax * 4);
@@ -793,18 +798,15 @@ extern "C" int hvk_descend(hvk_ctx* ctx, const hvk_tree* tree, const int16_t* d_
     const bool sel_lds = maxN <= kSelMaxLds;
     const size_t sel_bytes = sel_lds ? 2 * sizeof(float) * (size_t)maxN * kSelThreads : 0;
     // level 0 on the matrix cores when its operands fit the digit split
-    const bool dense = tree->n_roots > 0 && tree->n_roots <= kDenseN && tree->max_abs_proto > 0 &&
-                       tree->max_abs_proto < (int64_t(1) << 30) && q_maxabs <= 32639 &&
-                       K < 65536;  // each digit-weight class sums < 2 K 2^14 in int32
     timing_mark(ctx, 1, false, st);
     hipLaunchKernelGGL(desc_init, dim3(rb), dim3(kDescThreads), 0, st, bm[0], n_q, B);
     HVK_HIP(hipGetLastError());
     int cur = 0;
     for (int lvl = 0; lvl < L; ++lvl) {
         if (lvl == 0 && dense) {  // every read has one entry, the root
-            auto dense_k = tree->max_abs_proto <= 32639     ? desc_dense<2>
-                           : tree->max_abs_proto <= 8355711 ? desc_dense<3>
-                                                            : desc_dense<4>;
+            auto dense_k = pl.max_abs_proto <= 32639     ? desc_dense<2>
+                           : pl.max_abs_proto <= 8355711 ? desc_dense<3>
+                                                         : desc_dense<4>;
             hipLaunchKernelGGL(dense_k, dim3((n_q + 127) / 128), dim3(256), 0, st, t, d_q_counts,
                                q_ld, d_q_sqnorm, n_q, B, K, Fmax, sc);
         } else {
@@ -824,7 +826,7 @@ extern "C" int hvk_descend(hvk_ctx* ctx, const hvk_tree* tree, const int16_t* d_
                                    d_q_sqnorm, B, K, pairs, pkey, total, Fmax, sc);
             };
             // the longest candidate list of this level: the roots at level 0
-            const bool narrow = (lvl == 0 ? tree->n_roots : max_span) <= kDescGSmall;
+            const bool narrow = (lvl == 0 ? pl.n_roots : max_span) <= kDescGSmall;
             if (E == 8) {
                 if (narrow) score(desc_score<8, kDescGSmall>);
                 else score(desc_score<8, kDescG>);
@@ -848,6 +850,29 @@ extern "C" int hvk_descend(hvk_ctx* ctx, const hvk_tree* tree, const int16_t* d_
     HVK_HIP(hipGetLastError());
     timing_mark(ctx, 1, true, st);
     return HVK_OK;
+}
+
+}  // namespace hvk
+
+extern "C" int hvk_descend(hvk_ctx* ctx, const hvk_tree* tree, const int16_t* d_q_counts,
+                           const int64_t* d_q_sqnorm, int n_q, int64_t q_ld, int q_maxabs,
+                           int beam, float threshold, int levels, int32_t* d_path,
+                           float* d_path_score, int32_t* d_path_len, int32_t* d_exit_level,
+                           hvk_stream stream) {
+    if (!ctx || !tree || n_q < 0 || beam < 1 || beam > kMaxBeam || levels < 1 ||
+        levels > kMaxLevels || q_maxabs < 0)
+        return HVK_ERR_ARG;
+    if (n_q == 0) return HVK_OK;
+    if (!d_q_counts || !d_q_sqnorm || !d_path || !d_path_score || !d_path_len || !d_exit_level)
+        return HVK_ERR_ARG;
+    if (q_ld == 0) q_ld = ctx->ld;
+    if (q_ld < ctx->dim || q_ld % 128) return HVK_ERR_ARG;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    hipStream_t st = (hipStream_t)stream;
+    DescPlan pl;
+    if (int e = desc_prepare(ctx, tree, q_maxabs, beam, levels, pl, st)) return e;
+    return desc_launch(ctx, pl, d_q_counts, d_q_sqnorm, n_q, q_ld, threshold, d_path, d_path_score,
+                       d_path_len, d_exit_level, st);
 }
 
 extern "C" int hvk_tree_place(hvk_ctx* ctx, const int16_t* d_counts, int64_t counts_ld, int n,

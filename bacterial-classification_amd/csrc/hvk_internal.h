@@ -83,6 +83,39 @@ inline void timing_mark(hvk_ctx* c, int which, bool stop, hipStream_t s) {
     if (stop) c->ev_used[which] = true;
 }
 // Outputs of the encode kernel (each pointer nullable; ld = row stride).
+// A descent's tree on the device: prototype limbs and norms (the caller's)
+// and the candidate lists (uploaded into the context).
+struct TreeDev {
+    const int16_t* lo;
+    const int16_t* hi;
+    const int64_t* sqnorm;
+    int64_t ld;
+    int n_cat;
+    const int32_t* child_off;  // [n_cat + 1]
+    const int32_t* child_idx;
+    const int32_t* roots;
+    int n_roots;
+    const int32_t* row_maxabs;  // [n_cat] max |p|, nullable
+};
+
+// What desc_prepare checks and chooses for a tree, a read bound, beam and
+// levels; desc_launch runs a descent on it.
+struct DescPlan {
+    TreeDev t;
+    int E;  // elements per int32 partial-sum chunk (8 or 2)
+    int B, L;
+    int Fmax, maxN, max_span;
+    int n_cat, n_roots;
+    int64_t max_abs_proto;
+    bool dense;  // level 0 as the int8-digit GEMM
+};
+int desc_prepare(hvk_ctx* ctx, const hvk_tree* tree, int q_maxabs, int beam, int levels,
+                 DescPlan& pl, hipStream_t st);
+int desc_launch(hvk_ctx* ctx, const DescPlan& pl, const int16_t* d_q_counts,
+                const int64_t* d_q_sqnorm, int n_q, int64_t q_ld, float threshold,
+                int32_t* d_path, float* d_path_score, int32_t* d_path_len,
+                int32_t* d_exit_level, hipStream_t st);
+
 struct EncodeOut {
     int16_t* counts;
     int8_t* lo;

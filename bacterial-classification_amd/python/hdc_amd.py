@@ -31,7 +31,7 @@ EXPORTS = [
     "hvk_search_mfma_bounded", "hvk_prepare_counts", "hvk_ctx_set_tuning",
     "hvk_descend", "hvk_tree_place", "hvk_tree_sum", "hvk_tree_limbs",
     "hvk_batch_create", "hvk_batch_destroy", "hvk_batch_config", "hvk_batch_get_config",
-    "hvk_batch_classify", "hvk_batch_autotune",
+    "hvk_batch_classify", "hvk_batch_autotune", "hvk_batch_descend",
     "hvk_db_write", "hvk_db_open", "hvk_db_close", "hvk_db_info", "hvk_db_array", "hvk_db_upload",
     "hvk_search_mfma_bounded_noalloc",
 ]
@@ -157,6 +157,8 @@ def lib():
     L.hvk_batch_autotune.argtypes = [vp, vp, vp, vp, i32, ctypes.POINTER(Bundles), i32, vp, vp,
                                      i32, ctypes.POINTER(ctypes.c_float), vp,
                                      ctypes.POINTER(ctypes.c_int), vp]
+    L.hvk_batch_descend.argtypes = [vp, ctypes.POINTER(Tree), vp, vp, vp, i32, i32,
+                                    ctypes.c_float, i32, vp, vp, vp, vp, vp]
     _lib = L
     return L
 
@@ -666,6 +668,25 @@ class Batch:
                 table.append({"form": ["bounded", "two_limb"][int(f)], "shape": int(sh),
                               "splits": int(sp), "chunk": int(ch), "ms_per_mread": float(t)})
         return ms.value
+
+    def descend(self, tree: "TreeProto", seqs: DeviceSeqs, beam=5, threshold=0.0, levels=6,
+                out: "Descent | None" = None) -> "Descent":
+        """hvk_batch_descend: encode + descent per length-ordered chunk, replayed
+        from captured graphs; results in the reads' own rows."""
+        torch = _torch()
+        n = seqs.n
+        if out is None:
+            out = Descent(torch.full((n, levels), -1, dtype=torch.int32, device="cuda"),
+                          torch.zeros((n, levels), dtype=torch.float32, device="cuda"),
+                          torch.zeros(n, dtype=torch.int32, device="cuda"),
+                          torch.zeros(n, dtype=torch.int32, device="cuda"))
+        t = tree.struct()
+        check(lib().hvk_batch_descend(self.h, ctypes.byref(t), _ptr(seqs.packed),
+                                      _ptr(seqs.invalid), seqs.host_offsets.ctypes.data, n, beam,
+                                      threshold, levels, _ptr(out.path), _ptr(out.score),
+                                      _ptr(out.length), _ptr(out.exit_level), _stream()),
+              "hvk_batch_descend")
+        return out
 
 
 # ----------------------------------------------------------------------------

@@ -217,6 +217,9 @@ def parse():
     ap.add_argument("--batch-sample", type=int, default=1 << 20, help="autotuner sample")
     ap.add_argument("--descent-reads", type=int, default=1_000_000)
     ap.add_argument("--descent-beam", type=int, default=3)
+    ap.add_argument("--descent-chunk", type=int, default=262144,
+                    help="chunk of the batching form of the descent")
+    ap.add_argument("--no-batch-descend", action="store_true")
     ap.add_argument("--precision-reads", type=int, default=2_000_000)
     ap.add_argument("--precision-ancestors", type=int, default=5000)
     ap.add_argument("--copy-calib-mb", type=int, default=0,
@@ -637,6 +640,34 @@ def run_descent(a):
     ex = out.exit_level.cpu().numpy()
     acc = float((path[:, 5] == truth).mean())
 
+    # the batching form: hvk_batch_descend over length-ordered chunks replayed from graphs
+    batch_line = None
+    if not a.no_batch_descend:
+        bat = H.Batch(DIM, KMER, chunk=a.descent_chunk)
+        bout = H.Descent(torch.full((nq, 6), -1, dtype=torch.int32, device="cuda"),
+                         torch.zeros((nq, 6), dtype=torch.float32, device="cuda"),
+                         torch.zeros(nq, dtype=torch.int32, device="cuda"),
+                         torch.zeros(nq, dtype=torch.int32, device="cuda"))
+        bat.descend(tree, read_seqs, beam=beam, threshold=thr, levels=6, out=bout)  # capture
+        torch.cuda.synchronize()
+        tb = []
+        for _ in range(a.steps):
+            t = time.time()
+            bat.descend(tree, read_seqs, beam=beam, threshold=thr, levels=6, out=bout)
+            torch.cuda.synchronize()
+            tb.append(time.time() - t)
+        same_b = bool(torch.equal(bout.path, out.path) and torch.equal(bout.length, out.length) and
+                      torch.equal(bout.score.view(torch.int32), out.score.view(torch.int32)) and
+                      torch.equal(bout.exit_level, out.exit_level))
+        batch_line = {"reads_per_s": nq / float(np.mean(tb)), "ms_per_step": 1000.0 * float(np.mean(tb)),
+                      "chunk": a.descent_chunk, "equal_direct": same_b,
+                      "note": "hvk_batch_descend: encode + descent per length-ordered chunk, "
+                              "graph replays, results in input order"}
+        print(f"[descent] batch form: {nq / float(np.mean(tb)):.0f} reads/s, equal {same_b}",
+              file=sys.stderr, flush=True)
+        bat.close()
+        del bout
+
     cpu = None
     if not a.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -681,6 +712,7 @@ def run_descent(a):
         "prototype_build_s": build_s,
         "leaf_accuracy": acc, "exited_early": int((ex >= 0).sum()),
         "approx_int_ops_per_read": 2.0 * DIM * n_cand,
+        "batch_form": batch_line,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -411,6 +411,19 @@ int hvk_batch_autotune(hvk_batch* b, const uint32_t* d_packed, const uint32_t* d
                        const int64_t* h_offsets, int n, const hvk_bundles* db, int k,
                        int32_t* d_idx, float* d_score, int sample, float* ms_per_mread,
                        float* table, int* table_rows, hvk_stream stream);
+/* hvk_descend in the batching form: the reads (hvk_pack layout, device;
+ * h_offsets host) are ordered by length and cut into chunks; each chunk is
+ * encoded (int16 counts) and descended by launches captured into a hipGraph
+ * once and replayed, and its results go to the reads' own rows (outputs as
+ * hvk_descend's: d_path / d_path_score [n][levels], d_path_len,
+ * d_exit_level [n]).  The partial-sum bound uses the longest read's k-mer
+ * count.  Same results as hvk_encode_ex + hvk_descend.  The tree's device
+ * arrays are read at every replay (the candidate lists are uploaded by each
+ * call); they must stay valid while the call runs.  Enqueued on `stream`. */
+int hvk_batch_descend(hvk_batch* b, const hvk_tree* tree, const uint32_t* d_packed,
+                      const uint32_t* d_invalid, const int64_t* h_offsets, int n, int beam,
+                      float threshold, int levels, int32_t* d_path, float* d_path_score,
+                      int32_t* d_path_len, int32_t* d_exit_level, hvk_stream stream);
 
 /* ---- synthetic workloads (host, deterministic) -------------------------
  * Ancestors of uniform ACGT, descendants by substitution + indels, and reads

@@ -148,3 +148,40 @@ def test_batch_autotune(work):
     out = b.classify(seqs, dbb, TOPK, out=out)
     _same(out, direct)
     b.close()
+
+
+def test_batch_descend_equals_direct():
+    """hvk_batch_descend (length-ordered chunks of 256, each encoded and
+    descended from a captured graph) gives hvk_encode_ex + hvk_descend's paths,
+    scores (bit for bit), lengths and exit levels, on the first call and on a
+    replay into fresh outputs; reads include empty, short and all-N ones."""
+    import hdc_amd as H
+    import torch
+    D, k = 10240, 12
+    tax, rd, roff, truth = H.make_taxonomy(sizes=(4, 12, 24, 48, 96, 240), seq_len=1500,
+                                           n_reads=700, read_len=(150, 1600), seed=4)
+    reads = [rd[roff[i]:roff[i + 1]].decode() for i in range(len(roff) - 1)]
+    reads = reads[:300] + ["", "ACGTACGTAC", "N" * 300] + reads[300:]
+    buf, off = H.concat(reads)
+    seqs = H.DeviceSeqs(buf, off)
+    ctx = H.Context(D, k)
+    leaf_enc = H.encode(ctx, H.DeviceSeqs(tax.leaf_seqs, tax.leaf_offsets), counts=True,
+                        limbs=False, hi=False)
+    tree = H.build_prototypes(ctx, tax, leaf_enc.counts, tax.ranks[-1])
+    q = H.encode(ctx, seqs, counts=True, limbs=False, hi=False)
+    b = H.Batch(D, k, chunk=256)
+    for beam, thr in ((3, -1.0), (5, 0.0)):
+        print(f"batch descend beam={beam} thr={thr} reads={len(reads)}", flush=True)
+        want = H.descend(ctx, tree, q, beam=beam, threshold=thr, levels=6)
+        for rep in range(2):
+            got = b.descend(tree, seqs, beam=beam, threshold=thr, levels=6)
+            torch.cuda.synchronize()
+            assert torch.equal(got.length, want.length)
+            assert torch.equal(got.exit_level, want.exit_level)
+            for r in range(len(reads)):
+                n = int(want.length[r])
+                assert torch.equal(got.path[r, :n], want.path[r, :n]), f"read {r}"
+                assert torch.equal(got.score[r, :n].view(torch.int32),
+                                   want.score[r, :n].view(torch.int32)), f"read {r}"
+    b.close()
+    ctx.close()
