@@ -165,6 +165,10 @@ int hvk_ctx_set_tuning(hvk_ctx* ctx, int param, int value) {
             if (value < 0) return HVK_ERR_ARG;
             ctx->tune_select_kib = value;
             return HVK_OK;
+        case HVK_TUNE_ENCODER:
+            if (value < 0 || value > 1) return HVK_ERR_ARG;
+            ctx->encode_legacy = value == 1;
+            return HVK_OK;
         default:
             return HVK_ERR_ARG;
     }

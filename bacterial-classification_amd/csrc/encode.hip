@@ -73,6 +73,60 @@ __device__ __forceinline__ void add8(uint32_t (&planes)[P], const uint32_t (&x)[
     }
 }
 
+// a ^ b ^ c and the majority of a, b, c: one v_bitop3 each.
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+__device__ __forceinline__ uint32_t maj3(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0xE8);
+}
+
+// Carry-save adder: a + b + c = l + 2 h, bitwise.
+__device__ __forceinline__ void csa(uint32_t& h, uint32_t& l, uint32_t a, uint32_t b, uint32_t c) {
+    h = maj3(a, b, c);
+    l = xor3(a, b, c);
+}
+
+typedef __attribute__((address_space(3))) const uint32_t lds_u32;
+
+// LDS word at byte address (digit << (kPairShift + 2)) + base.  The digit is
+// wave uniform: one s_bfe extracts it and one v_lshl_add forms the address
+// (the compiler would shift and add on the scalar unit first).
+__device__ __forceinline__ uint32_t pair_row(uint32_t digit, uint32_t base);
+
+// planes += popcount-per-lane of x[0..15], Harley-Seal form: planes[0..3]
+// are the ones, twos, fours and eights of a carry-save accumulator and
+// planes[4..] a binary counter of sixteens.  The value held is
+// sum_p planes[p] 2^p in both readings, so the planes stay an ordinary
+// bit-sliced counter for the part merge and phase 3.  Per 16 words: 15
+// carry-save adders and one ripple of a single word into planes[4..].
+template <int P>
+__device__ __forceinline__ void add16(uint32_t (&planes)[P], const uint32_t (&x)[16]) {
+    static_assert(P >= 5, "add16 needs at least 5 planes");
+    uint32_t tA, tB, fA, fB, eA, eB, cy;
+    csa(tA, planes[0], planes[0], x[0], x[1]);
+    csa(tB, planes[0], planes[0], x[2], x[3]);
+    csa(fA, planes[1], planes[1], tA, tB);
+    csa(tA, planes[0], planes[0], x[4], x[5]);
+    csa(tB, planes[0], planes[0], x[6], x[7]);
+    csa(fB, planes[1], planes[1], tA, tB);
+    csa(eA, planes[2], planes[2], fA, fB);
+    csa(tA, planes[0], planes[0], x[8], x[9]);
+    csa(tB, planes[0], planes[0], x[10], x[11]);
+    csa(fA, planes[1], planes[1], tA, tB);
+    csa(tA, planes[0], planes[0], x[12], x[13]);
+    csa(tB, planes[0], planes[0], x[14], x[15]);
+    csa(fB, planes[1], planes[1], tA, tB);
+    csa(eB, planes[2], planes[2], fA, fB);
+    csa(cy, planes[3], planes[3], eA, eB);
+#pragma unroll
+    for (int p = 4; p < P; ++p) {
+        const uint32_t t = planes[p] & cy;
+        planes[p] ^= cy;
+        cy = t;
+    }
+}
+
 __device__ __forceinline__ int64_t wave_sum64(int64_t v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -181,22 +235,16 @@ __device__ __forceinline__ i16x2 as_i16x2(uint32_t v) { return __builtin_bit_cas
 __device__ __forceinline__ uint32_t as_u32(u16x2 v) { return __builtin_bit_cast(uint32_t, v); }
 __device__ __forceinline__ uint32_t as_u32(i16x2 v) { return __builtin_bit_cast(uint32_t, v); }
 
-// Phase 3, called by every thread of the workgroup: the threads with
-// `active` set own word w of the row and hold its bit-sliced counts; they
-// write counts / limbs / clamped rows, and the workgroup reduces the exact
-// sum of squares, max |count| and clamping residual of row s.
+// Phase 3 for one word: the thread owns word w of row s and holds its
+// bit-sliced counts; it writes the word's counts / limbs / clamped values and
+// adds the word's sum of squares, max |count| and clamping residual to sq,
+// mx and res.
 template <int P>
-__device__ __forceinline__ void finish_row(const uint32_t (&planes)[P], bool active, int w,
-                                           int nvalid, int D, int W32, int64_t s,
-                                           const EncodeOut& out, int64_t* red_sq, int* red_mx,
-                                           int* red_res) {
-    const int tid = threadIdx.x;
-    const int nthr = blockDim.x;
-    const int lane = tid & 63, wave = tid >> 6, nwaves = nthr >> 6;
-    int64_t sq = 0;
-    int mx = 0, res = 0;
+__device__ __forceinline__ void finish_word(const uint32_t (&planes)[P], int w, int nvalid, int D,
+                                            int64_t s, const EncodeOut& out, int64_t& sq, int& mx,
+                                            int& res) {
     const int64_t ld = out.ld;
-    if (active) {
+    {
         uint32_t packed16[16];
         uint32_t plo[8], phi[8], pcl[8];
         if constexpr (P <= 16) {
@@ -214,6 +262,7 @@ __device__ __forceinline__ void finish_row(const uint32_t (&planes)[P], bool act
             u16x2 mx2 = {0, 0};
             uint32_t res_u = 0;
             int sq32 = 0;
+            int64_t sq64 = 0;
 #pragma unroll
             for (int i = 0; i < 16; ++i) {
                 const u16x2 pc = as_u16x2(t[i]);
@@ -224,7 +273,7 @@ __device__ __forceinline__ void finish_row(const uint32_t (&planes)[P], bool act
                 }
                 const i16x2 cs = as_i16x2(as_u32(c));
                 if constexpr (P <= 11) sq32 = __builtin_amdgcn_sdot2(cs, cs, sq32, false);
-                else sq += (int64_t)(uint32_t)__builtin_amdgcn_sdot2(cs, cs, 0, false);
+                else sq64 += (int64_t)(uint32_t)__builtin_amdgcn_sdot2(cs, cs, 0, false);
                 const u16x2 zero = {0, 0};
                 const u16x2 a = as_u16x2(as_u32(__builtin_elementwise_max(cs, as_i16x2(as_u32(zero - c)))));
                 mx2 = __builtin_elementwise_max(mx2, a);
@@ -241,9 +290,10 @@ __device__ __forceinline__ void finish_row(const uint32_t (&planes)[P], bool act
                     pcl[i >> 1] = as_u32(cl);
                 }
             }
-            if constexpr (P <= 11) sq = sq32;
-            mx = max((int)mx2.x, (int)mx2.y);
-            res = (int)res_u;
+            if constexpr (P <= 11) sq += sq32;
+            else sq += sq64;
+            mx = max(mx, max((int)mx2.x, (int)mx2.y));
+            res += (int)res_u;
         } else {
 #pragma unroll
         for (int q = 0; q < 8; ++q) { plo[q] = 0; phi[q] = 0; pcl[q] = 0; }
@@ -292,6 +342,24 @@ __device__ __forceinline__ void finish_row(const uint32_t (&planes)[P], bool act
             dst[1] = make_uint4(pcl[4], pcl[5], pcl[6], pcl[7]);
         }
     }
+}
+
+// Phase 3, called by every thread of the workgroup: the threads with
+// `active` set own word w of the row and hold its bit-sliced counts; they
+// write counts / limbs / clamped rows, and the workgroup reduces the exact
+// sum of squares, max |count| and clamping residual of row s.
+template <int P>
+__device__ __forceinline__ void finish_row(const uint32_t (&planes)[P], bool active, int w,
+                                           int nvalid, int D, int W32, int64_t s,
+                                           const EncodeOut& out, int64_t* red_sq, int* red_mx,
+                                           int* red_res) {
+    const int tid = threadIdx.x;
+    const int nthr = blockDim.x;
+    const int lane = tid & 63, wave = tid >> 6, nwaves = nthr >> 6;
+    int64_t sq = 0;
+    int mx = 0, res = 0;
+    const int64_t ld = out.ld;
+    if (active) finish_word<P>(planes, w, nvalid, D, s, out, sq, mx, res);
     // zero padding columns beyond the last full word
     for (int64_t d = 32 * (int64_t)W32 + tid; d < ld; d += nthr) {
         const int64_t o = s * ld + d;
@@ -396,12 +464,23 @@ encode_kernel(const uint32_t* __restrict__ gtable, int k_rt, int D, int W32,
 
 // Pair-table encoder: the table of base PAIRS T2[c_2j + 4 c_2j+1][j][w] =
 // rot(base[c_2j], 2j) ^ rot(base[c_2j+1], 2j+1) halves the LDS lookups and
-// address adds per k-mer (6 instead of 12 at k = 12).  The table is 4x the
-// single one (123 KB at D = 10240, k = 12), so one workgroup fills a CU; to
-// keep more waves per CU, NP parts of the workgroup (as many as 1024 threads
-// hold, at most 4) share each sequence: they take alternate blocks of 8
-// k-mers into their own bit-sliced counters, which are added (bit-sliced)
-// into part 0's before phase 3.
+// address adds per k-mer (6 instead of 12 at k = 12).  The 16 entries sit
+// kPairStride words apart (128 KB), so an entry's offset is a shift of its
+// digit folded into the lane's address add (no scalar multiply).  One
+// workgroup fills a CU; to keep more waves per CU, NP parts of the workgroup
+// (as many as 1024 threads hold, at most 4) share each sequence: they take
+// alternate blocks of 16 k-mers into their own Harley-Seal counters, which
+// are added (bit-sliced) into part 0's before phase 3.
+constexpr int kPairShift = 11;
+constexpr int kPairStride = 1 << kPairShift;  // words per pair-table entry
+constexpr int kPairBlock = 16;                // k-mers per block of a part
+
+__device__ __forceinline__ uint32_t pair_row(uint32_t digit, uint32_t base) {
+    uint32_t a;
+    asm("v_lshl_add_u32 %0, %1, %2, %3" : "=v"(a) : "s"(digit), "i"(kPairShift + 2), "v"(base));
+    return *(lds_u32*)(size_t)a;
+}
+
 template <int K, int P>
 __global__ void __launch_bounds__(1024)
 encode_pair_kernel(const uint32_t* __restrict__ gtable, int k_rt, int D, int W32, int TW, int NP,
@@ -412,8 +491,8 @@ encode_pair_kernel(const uint32_t* __restrict__ gtable, int k_rt, int D, int W32
     const int k = K > 0 ? K : k_rt;
     const int KH = k / 2;
     extern __shared__ uint32_t smem[];
-    uint32_t* tab2 = smem;                                  // [16][KH][W32]
-    uint32_t* tab1 = tab2 + 16 * KH * W32;                  // [4][W32], odd k: last base
+    uint32_t* tab2 = smem;                                  // [16][kPairStride]: [KH][W32]
+    uint32_t* tab1 = tab2 + 16 * kPairStride;               // [4][W32], odd k: last base
     uint32_t* codes = tab1 + ((k & 1) ? 4 * W32 : 0);      // [kChunk]
     uint32_t* xplanes = codes + kChunk;                     // [P][W32]
     __shared__ int64_t red_sq[16];
@@ -427,10 +506,11 @@ encode_pair_kernel(const uint32_t* __restrict__ gtable, int k_rt, int D, int W32
     const int w = tid - part * TW;
     const int kW = k * W32;
     const int KHW = KH * W32;
+    const uint32_t lds_w = (uint32_t)(size_t)(lds_u32*)tab2 + 4u * w;  // LDS byte address
     for (int i = tid; i < 16 * KHW; i += nthr) {
         const int v = i / KHW, rem = i - v * KHW, j = rem / W32, ww = rem - j * W32;
-        tab2[i] = gtable[(v & 3) * kW + (2 * j) * W32 + ww] ^
-                  gtable[(v >> 2) * kW + (2 * j + 1) * W32 + ww];
+        tab2[v * kPairStride + rem] = gtable[(v & 3) * kW + (2 * j) * W32 + ww] ^
+                                      gtable[(v >> 2) * kW + (2 * j + 1) * W32 + ww];
     }
     if (k & 1)
         for (int i = tid; i < 4 * W32; i += nthr) {
@@ -467,37 +547,68 @@ encode_pair_kernel(const uint32_t* __restrict__ gtable, int k_rt, int D, int W32
         for (int64_t c0 = 0; c0 < npos; c0 += kChunk) {
             const int cn = (int)((npos - c0) < kChunk ? (npos - c0) : kChunk);
             __syncthreads();
-            const int cn8 = (cn + 7) & ~7;
-            for (int j = tid; j < cn8; j += nthr)
+            const int cnb = (cn + kPairBlock - 1) & ~(kPairBlock - 1);
+            for (int j = tid; j < cnb; j += nthr)
                 codes[j] = (c0 == 0 && j == tid && pre)
                                ? code_of_windows(pw, iv, off + j, k, mask2k, maskk)
                                : kmer_code(packed, invalid, off + c0 + j, j, cn, k, mask2k, maskk);
             __syncthreads();
             if (w < W32) {
-                for (int j = 8 * part; j < cn; j += 8 * NP) {  // alternate 8-blocks per part
+                // alternate 16-blocks per part
+                for (int j = kPairBlock * part; j < cn; j += kPairBlock * NP) {
                     // branch-free: an invalid code (all digits 3) reads valid
                     // table rows and its word is dropped afterwards
-                    uint32_t c[8], x[8];
-                    load_codes8(codes, j, c);
+                    uint32_t c[16], x[16];
+                    load_codes8(codes, j, *reinterpret_cast<uint32_t(*)[8]>(&c[0]));
+                    load_codes8(codes, j + 8, *reinterpret_cast<uint32_t(*)[8]>(&c[8]));
+                    if constexpr (K == 12 || K == 10) {
+                        // software pipeline: the rows of k-mer u + 1 are
+                        // requested before the rows of k-mer u are combined
+                        // (the scheduling fences keep the extracted digits
+                        // from piling up in SGPRs)
+                        constexpr int KH_ = K / 2;
+                        uint32_t r[2][KH_];
+                        auto issue = [&](int u, uint32_t (&rr)[KH_]) {
+                            const uint32_t code = __builtin_amdgcn_readfirstlane(c[u]);
 #pragma unroll
-                    for (int u = 0; u < 8; ++u) {
+                            for (int jj = 0; jj < KH_; ++jj)
+                                rr[jj] = pair_row((code >> (4 * jj)) & 15u, lds_w + 4u * jj * W32);
+                        };
+                        issue(0, r[0]);
+#pragma unroll
+                        for (int u = 0; u < kPairBlock; ++u) {
+                            if (u + 1 < kPairBlock) issue(u + 1, r[(u + 1) & 1]);
+                            __builtin_amdgcn_sched_barrier(0);
+                            const uint32_t(&q)[KH_] = r[u & 1];
+                            uint32_t y;
+                            if constexpr (K == 12) y = xor3(xor3(q[0], q[1], q[2]), q[3], q[4]) ^ q[5];
+                            else y = xor3(xor3(q[0], q[1], q[2]), q[3], q[4]);
+                            const bool valid = __builtin_amdgcn_readfirstlane(c[u]) != kInvalidCode;
+                            nvalid += valid ? 1 : 0;
+                            x[u] = valid ? y : 0u;
+                        }
+                    } else {
+#pragma unroll
+                    for (int u = 0; u < kPairBlock; ++u) {
                         const uint32_t code = __builtin_amdgcn_readfirstlane(c[u]);
+                        auto row = [&](int jj) {
+                            return pair_row((code >> (4 * jj)) & 15u, lds_w + 4u * jj * W32);
+                        };
                         uint32_t y = 0;
-                        if (K > 0) {
+                        if constexpr (K > 0) {
 #pragma unroll
-                            for (int jj = 0; jj < (K > 0 ? K / 2 : 1); ++jj)
-                                y ^= tab2[((code >> (4 * jj)) & 15u) * KHW + jj * W32 + w];
+                            for (int jj = 0; jj < (K > 0 ? K / 2 : 1); ++jj) y ^= row(jj);
                             if (K & 1) y ^= tab1[((code >> (2 * (k - 1))) & 3u) * W32 + w];
                         } else {
-                            for (int jj = 0; jj < KH; ++jj)
-                                y ^= tab2[((code >> (4 * jj)) & 15u) * KHW + jj * W32 + w];
+                            for (int jj = 0; jj < KH; ++jj) y ^= row(jj);
                             if (k & 1) y ^= tab1[((code >> (2 * (k - 1))) & 3u) * W32 + w];
                         }
                         const bool valid = code != kInvalidCode;
                         nvalid += valid ? 1 : 0;
                         x[u] = valid ? y : 0u;
                     }
-                    add8<P>(planes, x);
+                    }
+                    add16<P>(planes, x);
                 }
             }
         }
@@ -533,6 +644,175 @@ encode_pair_kernel(const uint32_t* __restrict__ gtable, int k_rt, int D, int W32
     }
 }
 
+// One wave per sequence (encode_wave_kernel), for even k in {10, 12} and
+// 257..320 or up to 256 words of 32 dimensions.
+//
+// Each wave64 LDS read costs about the same number of LDS cycles whatever
+// its width (measured on gfx950: 3.9 clocks for 4 bytes per lane, 6.2 for
+// 16), so the 4-byte pair-table reads of the kernel above keep the LDS busy
+// while the VALU idles.  Here lane l owns the five words 4l .. 4l+3 (one
+// 16-byte read per lookup) and 256 + l (one 4-byte read), and a wave holds a
+// whole sequence: 12 LDS reads per k-mer instead of 30, and no workgroup
+// barrier after the pair table is staged.  The k-mer words go into the
+// Harley-Seal counters as they arrive (ones, twos, fours, eights in
+// planes[0..3], sixteens rippled into planes[4..]), so only the carries of a
+// 16-k-mer block are live.
+constexpr int kWaveChunk = 512;  // k-mer positions per chunk (codes per wave)
+constexpr int kWaveWaves = 8;    // waves (sequences in flight) per workgroup
+
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ u32x4 pair_row4(uint32_t digit, uint32_t base) {
+    uint32_t a;
+    asm("v_lshl_add_u32 %0, %1, %2, %3" : "=v"(a) : "s"(digit), "i"(kPairShift + 2), "v"(base));
+    return *(const __attribute__((address_space(3))) u32x4*)(size_t)a;
+}
+
+template <int K, int P, int NB32>
+__global__ void __launch_bounds__(64 * kWaveWaves)
+encode_wave_kernel(const uint32_t* __restrict__ gtable, int D, int W32,
+                   const uint32_t* __restrict__ packed, const uint32_t* __restrict__ invalid,
+                   const int64_t* __restrict__ offsets, const int64_t* __restrict__ starts,
+                   const int32_t* __restrict__ lens, int window_len, int n_seqs,
+                   EncodeOut out) {
+    static_assert(K == 10 || K == 12, "even k with a fixed pair count");
+    constexpr int KH = K / 2;
+    constexpr int NW = 4 + NB32;             // words per lane
+    constexpr int W32P = 256 + 64 * NB32;    // table row length (words)
+    static_assert(KH * W32P <= kPairStride, "pair-table entry overflows its stride");
+    extern __shared__ uint32_t smem[];
+    uint32_t* tab2 = smem;  // [16][kPairStride]: [KH][W32P], words >= W32 zero
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    uint32_t* codes = smem + 16 * kPairStride + wave * kWaveChunk;
+    const int kW = K * W32;
+    for (int i = tid; i < 16 * KH * W32P; i += blockDim.x) {
+        const int v = i / (KH * W32P), rem = i - v * (KH * W32P), j = rem / W32P,
+                  ww = rem - j * W32P;
+        tab2[v * kPairStride + rem] =
+            ww < W32 ? gtable[(v & 3) * kW + (2 * j) * W32 + ww] ^
+                           gtable[(v >> 2) * kW + (2 * j + 1) * W32 + ww]
+                     : 0u;
+    }
+    __syncthreads();
+
+    constexpr uint32_t mask2k = (1u << (2 * K)) - 1u, maskk = (1u << K) - 1u;
+    const uint32_t tab_lds = (uint32_t)(size_t)(lds_u32*)tab2;
+    const uint32_t a128 = tab_lds + 16u * lane;          // words 4 lane .. 4 lane + 3
+    const uint32_t a32 = tab_lds + 4u * (256 + lane);    // word 256 + lane
+    const int64_t ld = out.ld;
+    const int stride = gridDim.x * kWaveWaves;
+    for (int s = blockIdx.x * kWaveWaves + wave; s < n_seqs; s += stride) {
+        const int64_t off = starts ? starts[s] : offsets[s];
+        const int64_t len = starts ? (lens ? (int64_t)lens[s] : (int64_t)window_len)
+                                   : offsets[s + 1] - off;
+        const int64_t npos = len - K + 1;
+        uint32_t planes[P][NW];
+#pragma unroll
+        for (int p = 0; p < P; ++p)
+#pragma unroll
+            for (int g = 0; g < NW; ++g) planes[p][g] = 0;
+        int nvalid = 0;
+
+        for (int64_t c0 = 0; c0 < npos; c0 += kWaveChunk) {
+            const int cn = (int)((npos - c0) < kWaveChunk ? (npos - c0) : kWaveChunk);
+            const int cnb = (cn + 15) & ~15;
+            // the previous chunk's code reads are done before they are overwritten
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            for (int j = lane; j < cnb; j += 64)
+                codes[j] = kmer_code(packed, invalid, off + c0 + j, j, cn, K, mask2k, maskk);
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            for (int j = 0; j < cn; j += 16) {
+                uint32_t c[16];
+                load_codes8(codes, j, *reinterpret_cast<uint32_t(*)[8]>(&c[0]));
+                load_codes8(codes, j + 8, *reinterpret_cast<uint32_t(*)[8]>(&c[8]));
+                uint32_t r[2][KH][NW];
+                auto issue = [&](int u, uint32_t (&rr)[KH][NW]) {
+                    const uint32_t code = __builtin_amdgcn_readfirstlane(c[u]);
+#pragma unroll
+                    for (int jj = 0; jj < KH; ++jj) {
+                        const uint32_t e = (code >> (4 * jj)) & 15u;
+                        const u32x4 v = pair_row4(e, a128 + 4u * jj * W32P);
+                        rr[jj][0] = v.x; rr[jj][1] = v.y; rr[jj][2] = v.z; rr[jj][3] = v.w;
+                        if constexpr (NB32) rr[jj][4] = pair_row(e, a32 + 4u * jj * W32P);
+                    }
+                };
+                uint32_t y0[NW], tA[NW], fA[NW], eA[NW];
+                issue(0, r[0]);
+#pragma unroll
+                for (int u = 0; u < 16; ++u) {
+                    if (u + 1 < 16) issue(u + 1, r[(u + 1) & 1]);
+                    __builtin_amdgcn_sched_barrier(0);
+                    const uint32_t(&q)[KH][NW] = r[u & 1];
+                    const bool valid = __builtin_amdgcn_readfirstlane(c[u]) != kInvalidCode;
+                    nvalid += valid ? 1 : 0;
+                    uint32_t y[NW];
+#pragma unroll
+                    for (int g = 0; g < NW; ++g) {
+                        uint32_t v = xor3(xor3(q[0][g], q[1][g], q[2][g]), q[3][g], q[4][g]);
+                        if constexpr (K == 12) v ^= q[5][g];
+                        y[g] = valid ? v : 0u;
+                    }
+                    if ((u & 1) == 0) {
+#pragma unroll
+                        for (int g = 0; g < NW; ++g) y0[g] = y[g];
+                        continue;
+                    }
+                    // Harley-Seal step on the pair (u - 1, u)
+#pragma unroll
+                    for (int g = 0; g < NW; ++g) {
+                        uint32_t t, f, e8, cy;
+                        csa(t, planes[0][g], planes[0][g], y0[g], y[g]);
+                        if ((u & 3) == 1) { tA[g] = t; continue; }
+                        csa(f, planes[1][g], planes[1][g], tA[g], t);
+                        if ((u & 7) == 3) { fA[g] = f; continue; }
+                        csa(e8, planes[2][g], planes[2][g], fA[g], f);
+                        if (u == 7) { eA[g] = e8; continue; }
+                        csa(cy, planes[3][g], planes[3][g], eA[g], e8);
+#pragma unroll
+                        for (int p = 4; p < P; ++p) {
+                            const uint32_t tt = planes[p][g] & cy;
+                            planes[p][g] ^= cy;
+                            cy = tt;
+                        }
+                    }
+                }
+            }
+        }
+
+        // phase 3: the lane's five (or four) words, then the row reductions
+        int64_t sq = 0;
+        int mx = 0, res = 0;
+#pragma unroll
+        for (int g = 0; g < NW; ++g) {
+            const int w = g < 4 ? 4 * lane + g : 256 + lane;
+            if (w < W32) {
+                uint32_t pl[P];
+#pragma unroll
+                for (int p = 0; p < P; ++p) pl[p] = planes[p][g];
+                finish_word<P>(pl, w, nvalid, D, s, out, sq, mx, res);
+            }
+        }
+        for (int64_t d = 32 * (int64_t)W32 + lane; d < ld; d += 64) {
+            const int64_t o = s * ld + d;
+            if (out.counts) out.counts[o] = 0;
+            if (out.lo) out.lo[o] = 0;
+            if (out.hi) out.hi[o] = 0;
+            if (out.clamp8) out.clamp8[o] = 0;
+        }
+        sq = wave_sum64(sq);
+        mx = wave_max(mx);
+        res = wave_sum(res);
+        if (lane == 0) {
+            if (out.sqnorm) out.sqnorm[s] = sq;
+            if (out.maxabs) out.maxabs[s] = mx;
+            if (out.nkmers) out.nkmers[s] = nvalid;
+            if (out.resid1) out.resid1[s] = res;
+        }
+    }
+}
+
 template <int K, int P>
 int launch_kp(hvk_ctx* ctx, const uint32_t* packed, const uint32_t* invalid,
               const int64_t* offsets, const int64_t* starts, const int32_t* lens, int window_len,
@@ -540,14 +820,40 @@ int launch_kp(hvk_ctx* ctx, const uint32_t* packed, const uint32_t* invalid,
     const int W32 = ctx->w32;
     const int threads = (W32 + 63) / 64 * 64;
     const int k = ctx->k;
-    // pair table when its 16 x (k/2) rows, the codes and one part's counters
-    // fit one workgroup; as many parts (<= 4) as 1024 threads hold
-    const size_t lds_pair = sizeof(uint32_t) * ((size_t)16 * (k / 2) * W32 +
-                                                ((k & 1) ? 4 * (size_t)W32 : 0) + kChunk +
-                                                (size_t)P * W32);
-    const int np = std::min(4, 1024 / threads);
-    const bool pair = k >= 2 && np >= 1 && lds_pair + 512 <= (size_t)ctx->max_lds;
+    // pair table when an entry's k/2 rows fit its kPairStride words and the
+    // table, the codes and (with more than one part) one part's counters fit
+    // one workgroup; as many parts (<= 4) as 1024 threads hold, else one
+    const size_t lds_one = sizeof(uint32_t) * ((size_t)16 * kPairStride +
+                                               ((k & 1) ? 4 * (size_t)W32 : 0) + kChunk);
+    const size_t lds_parts = lds_one + sizeof(uint32_t) * (size_t)P * W32;
+    int np = std::min(4, 1024 / threads);
+    if (np > 1 && lds_parts + 512 > (size_t)ctx->max_lds) np = 1;
+    const size_t lds_pair = np > 1 ? lds_parts : lds_one;
+    const bool pair = k >= 2 && np >= 1 && (size_t)(k / 2) * W32 <= (size_t)kPairStride &&
+                      lds_pair + 512 <= (size_t)ctx->max_lds;
     int grid = n < 4 * ctx->n_cu ? n : 4 * ctx->n_cu;
+    if constexpr ((K == 10 || K == 12) && P <= 16) {
+        // one wave per sequence when a lane's four (or five) words cover the row
+        const size_t lds_wave = sizeof(uint32_t) * ((size_t)16 * kPairStride +
+                                                    (size_t)kWaveWaves * kWaveChunk);
+        const int nb32 = W32 <= 256 ? 0 : (W32 <= 320 ? 1 : -1);
+        if (nb32 >= 0 && W32 > 128 && lds_wave + 512 <= (size_t)ctx->max_lds &&
+            !ctx->encode_legacy) {
+            const int wgrid = std::max(1, std::min(ctx->n_cu, (n + kWaveWaves - 1) / kWaveWaves));
+            timing_mark(ctx, 0, false, st);
+            if (nb32)
+                hipLaunchKernelGGL((encode_wave_kernel<K, P, 1>), dim3(wgrid),
+                                   dim3(64 * kWaveWaves), lds_wave, st, ctx->d_table, ctx->dim,
+                                   W32, packed, invalid, offsets, starts, lens, window_len, n, out);
+            else
+                hipLaunchKernelGGL((encode_wave_kernel<K, P, 0>), dim3(wgrid),
+                                   dim3(64 * kWaveWaves), lds_wave, st, ctx->d_table, ctx->dim,
+                                   W32, packed, invalid, offsets, starts, lens, window_len, n, out);
+            timing_mark(ctx, 0, true, st);
+            HVK_HIP(hipGetLastError());
+            return HVK_OK;
+        }
+    }
     timing_mark(ctx, 0, false, st);
     if (pair) {
         hipLaunchKernelGGL((encode_pair_kernel<K, P>), dim3(grid), dim3(np * threads), lds_pair,

@@ -44,6 +44,7 @@ BATCH_FORM_TWO_LIMB = 1
 TUNE_SPLITS = 1
 TUNE_SELECT_KIB = 2
 TUNE_SHAPE = 3
+TUNE_ENCODER = 4
 
 
 class HvkError(RuntimeError):

@@ -70,8 +70,9 @@ int hvk_base_hv(const hvk_ctx* ctx, char symbol, uint64_t* host_out);
 int hvk_ctx_set_timing(hvk_ctx* ctx, int enable);
 int hvk_ctx_last_timing(hvk_ctx* ctx, float* encode_ms, float* search_ms);
 
-/* Tuning of the search launches (no counterpart in the reference; used by
- * the tile autotuner and the tests).  Results do not depend on it.
+/* Tuning of the encode and search launches (no counterpart in the
+ * reference; used by the tile autotuner and the tests).  Results do not
+ * depend on it.
  * HVK_TUNE_SPLITS: number of reference slices per query tile of the MFMA
  * search, 0 = automatic (the default), else 1..64.
  * HVK_TUNE_SELECT_KIB: workspace budget (KiB) of the selection path for
@@ -79,10 +80,13 @@ int hvk_ctx_last_timing(hvk_ctx* ctx, float* encode_ms, float* search_ms);
  * HVK_TUNE_SHAPE: tile of hvk_search_mfma_bounded for k <= 5, as queries x
  * references per workgroup: 0 = 256 x 256 (default), 1 = 128 x 256,
  * 2 = 256 x 128, 3 = 128 x 512, 4 = 512 x 128, 5 = 256 x 256 on a 4 x 2
- * wave grid. */
+ * wave grid.
+ * HVK_TUNE_ENCODER: 0 = automatic (the default: one wave per sequence where
+ * it applies), 1 = the workgroup-per-sequence encoders only. */
 #define HVK_TUNE_SPLITS 1
 #define HVK_TUNE_SELECT_KIB 2
 #define HVK_TUNE_SHAPE 3
+#define HVK_TUNE_ENCODER 4
 int hvk_ctx_set_tuning(hvk_ctx* ctx, int param, int value);
 
 /* ---- packing (host) -----------------------------------------------------

@@ -7,16 +7,24 @@ import seqgen
 
 pytestmark = pytest.mark.gpu
 
-# Pair-table kernel: (10000, 5) three parts, (8192, 10) four parts, (10240, 12)
-# three parts, (20000, 7) one part.  (20000, 12): the pair table does not fit
-# in LDS, so the single-table kernel runs.
-CASES = [(10000, 5), (8192, 10), (10240, 12), (20000, 7), (20000, 12)]
+# One wave per sequence (k in {10, 12}, 129..320 words of 32 dims): (8192, 10)
+# four words per lane, (10240, 12) and (10000, 12) five (the fifth word of
+# lanes 57..63 past D=10000), (5000, 12) four words with lanes past the row.
+# The same (D, k) with legacy=True and (10000, 5), (20000, 7) run the
+# workgroup-per-sequence pair-table kernel ((10000, 5) three parts, (8192, 10)
+# four, (10240, 12) three, (20000, 7) one).  (20000, 12): the pair table does
+# not fit in LDS, so the single-table kernel runs.
+CASES = [(10000, 5, False), (8192, 10, False), (8192, 10, True), (10240, 12, False),
+         (10240, 12, True), (10000, 12, False), (5000, 12, False), (20000, 7, False),
+         (20000, 12, False)]
 
 
-def _encode(dim, k, seqs, **kw):
+def _encode(dim, k, seqs, legacy=False, **kw):
     import hdc_amd as H
     import torch
     ctx = H.Context(dim, k)
+    if legacy:
+        ctx.set_tuning(H.TUNE_ENCODER, 1)
     buf, off = H.concat(seqs)
     dev = H.DeviceSeqs(buf, off)
     enc = H.encode(ctx, dev, **kw)
@@ -24,12 +32,15 @@ def _encode(dim, k, seqs, **kw):
     return ctx, enc
 
 
-@pytest.mark.parametrize("dim,k", CASES)
-def test_counts_bit_exact(dim, k):
+@pytest.mark.parametrize("dim,k,legacy", CASES)
+def test_counts_bit_exact(dim, k, legacy):
     seqs = seqgen.mixed_set(1000 + k, 24, 1, 400)
     seqs += ["", "A" * (k - 1), "ACGT" * 3, "N" * 50, "acgtacgtacgtacgtacgt", "ACGTN" * 40]
-    print(f"encode D={dim} k={k} n={len(seqs)}", flush=True)
-    ctx, enc = _encode(dim, k, seqs)
+    # chunk edges of both kernels (512 and 1024 k-mer positions)
+    rng = np.random.default_rng(dim + k)
+    seqs += [seqgen.random_seq(rng, n + k - 1) for n in (511, 512, 513, 1024, 1025, 1600)]
+    print(f"encode D={dim} k={k} legacy={legacy} n={len(seqs)}", flush=True)
+    ctx, enc = _encode(dim, k, seqs, legacy)
     counts = enc.counts.cpu().numpy()
     lo = enc.lo.cpu().numpy().astype(np.int32)
     hi = enc.hi.cpu().numpy().astype(np.int32)
@@ -46,11 +57,12 @@ def test_counts_bit_exact(dim, k):
         assert nk[i] == n
 
 
-def test_long_sequences_planes():
+@pytest.mark.parametrize("legacy", [False, True])
+def test_long_sequences_planes(legacy):
     """> 2048 k-mers (16 counter planes) and the int32 oracle sum."""
     rng = np.random.default_rng(7)
     seqs = [seqgen.random_seq(rng, 5000), "A" * 3000, seqgen.random_seq(rng, 2100)]
-    ctx, enc = _encode(10240, 12, seqs)
+    ctx, enc = _encode(10240, 12, seqs, legacy)
     counts = enc.counts.cpu().numpy()
     for i, s in enumerate(seqs):
         ref32, _ = orc.compute32(10240, 12, s)
@@ -81,12 +93,13 @@ def test_base_hv_matches_oracle():
             assert np.array_equal(ctx.base_hv(ch), orc.base_hv(dim, ch))
 
 
-@pytest.mark.parametrize("dim,k", [(10240, 12), (10000, 5)])
-def test_clamp_and_residual_outputs(dim, k):
+@pytest.mark.parametrize("dim,k,legacy", [(10240, 12, False), (10240, 12, True),
+                                          (10000, 5, False)])
+def test_clamp_and_residual_outputs(dim, k, legacy):
     """clamp8 = clip(count, -127, 127); resid1 = sum |count - clamp8|."""
     rng = np.random.default_rng(21)
     seqs = [seqgen.random_seq(rng, 1500), "A" * 400, seqgen.random_seq(rng, 200), "ACGTAC" * 100]
-    ctx, enc = _encode(dim, k, seqs, clamp=True)
+    ctx, enc = _encode(dim, k, seqs, legacy, clamp=True)
     c = enc.counts.cpu().numpy().astype(np.int32)[:, :dim]
     cl = enc.clamp8.cpu().numpy().astype(np.int32)
     want = np.clip(c, -127, 127)
@@ -96,14 +109,17 @@ def test_clamp_and_residual_outputs(dim, k):
     assert enc.resid1.cpu().numpy()[1] > 0  # poly-A saturates
 
 
-@pytest.mark.parametrize("dim,k", [(10000, 5), (10240, 12)])
-def test_windows_match_substring_compute(dim, k):
+@pytest.mark.parametrize("dim,k,legacy", [(10000, 5, False), (10240, 12, False),
+                                          (10240, 12, True)])
+def test_windows_match_substring_compute(dim, k, legacy):
     """hvk_encode_windows == compute() of each window's substring."""
     import hdc_amd as H
     import torch
     seqs = seqgen.mixed_set(55, 5, 300, 700)
     buf, off = H.concat(seqs)
     ctx = H.Context(dim, k)
+    if legacy:
+        ctx.set_tuning(H.TUNE_ENCODER, 1)
     dev = H.DeviceSeqs(buf, off)
     starts = []
     for i, s in enumerate(seqs):
