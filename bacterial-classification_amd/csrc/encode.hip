@@ -213,21 +213,27 @@ __device__ __forceinline__ uint32_t unzip_bits(uint32_t x) {
     return x;
 }
 
+// One stage of the transposes below: swap the SH x SH sub-blocks.  A stage
+// per template instance, so every register index is a constant (a loop over
+// the stages stays rolled and indexes registers at run time).
+template <int SH>
+__device__ __forceinline__ void transpose_stage(uint32_t (&w)[16], uint32_t m) {
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        if (j & SH) continue;
+        const uint32_t t = ((w[j] >> SH) ^ w[j + SH]) & m;
+        w[j + SH] ^= t;
+        w[j] ^= t << SH;
+    }
+}
+
 // Two 16 x 16 bit-matrix transposes at once (one per 16-bit half): on
 // return bit p of half h of w[i] is bit i of half h of the input w[p].
 __device__ __forceinline__ void transpose16x2(uint32_t (&w)[16]) {
-#pragma unroll
-    for (int sh = 8; sh >= 1; sh >>= 1) {
-        const uint32_t m = sh == 8 ? 0x00FF00FFu : sh == 4 ? 0x0F0F0F0Fu
-                         : sh == 2 ? 0x33333333u : 0x55555555u;
-#pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            if (j & sh) continue;
-            const uint32_t t = ((w[j] >> sh) ^ w[j + sh]) & m;
-            w[j + sh] ^= t;
-            w[j] ^= t << sh;
-        }
-    }
+    transpose_stage<8>(w, 0x00FF00FFu);
+    transpose_stage<4>(w, 0x0F0F0F0Fu);
+    transpose_stage<2>(w, 0x33333333u);
+    transpose_stage<1>(w, 0x55555555u);
 }
 
 __device__ __forceinline__ u16x2 as_u16x2(uint32_t v) { return __builtin_bit_cast(u16x2, v); }
@@ -256,6 +262,7 @@ __device__ __forceinline__ void finish_word(const uint32_t (&planes)[P], int w, 
             for (int p = 0; p < 16; ++p) t[p] = p < P ? unzip_bits(planes[p]) : 0u;
             transpose16x2(t);
             const int lim = D - 32 * w;  // dims >= lim of this word are padding
+            const bool tail = (D & 31) != 0 && lim < 32;  // uniform test first
             const u16x2 nv2 = {(unsigned short)nvalid, (unsigned short)nvalid};
             const u16x2 c127 = {127, 127}, c128 = {128, 128}, ones = {1, 1};
             const i16x2 p127 = {127, 127}, m127 = {-127, -127};
@@ -267,7 +274,7 @@ __device__ __forceinline__ void finish_word(const uint32_t (&planes)[P], int w, 
             for (int i = 0; i < 16; ++i) {
                 const u16x2 pc = as_u16x2(t[i]);
                 u16x2 c = nv2 - (pc + pc);
-                if (lim < 32) {
+                if (tail) {
                     if (2 * i >= lim) c.x = 0;
                     if (2 * i + 1 >= lim) c.y = 0;
                 }
@@ -656,9 +663,29 @@ encode_pair_kernel(const uint32_t* __restrict__ gtable, int k_rt, int D, int W32
 // barrier after the pair table is staged.  The k-mer words go into the
 // Harley-Seal counters as they arrive (ones, twos, fours, eights in
 // planes[0..3], sixteens rippled into planes[4..]), so only the carries of a
-// 16-k-mer block are live.
+// 16-k-mer block are live.  Phase 1 writes the pair digits in 5-bit fields;
+// a k-mer with a base other than A/C/G/T (and the padding past a chunk)
+// points every field at entry 16, an all-zero row, so it adds nothing and
+// needs no select.  Phase 1 also counts the valid k-mers.
 constexpr int kWaveChunk = 512;  // k-mer positions per chunk (codes per wave)
 constexpr int kWaveWaves = 8;    // waves (sequences in flight) per workgroup
+constexpr int kZeroEntry = 16;   // pair-table entry of zeros
+
+// The 2-bit digits of a canonical code (digit i = c_i) as KH 5-bit pair
+// fields (field j = c_2j + 4 c_2j+1); kInvalidCode maps to kZeroEntry in
+// every field.
+template <int KH>
+__device__ __forceinline__ uint32_t pair_fields(uint32_t code) {
+    uint32_t f = 0;
+#pragma unroll
+    for (int j = 0; j < KH; ++j) f |= ((code >> (4 * j)) & 15u) << (5 * j);
+    constexpr uint32_t zero = [] {
+        uint32_t z = 0;
+        for (int j = 0; j < KH; ++j) z |= (uint32_t)kZeroEntry << (5 * j);
+        return z;
+    }();
+    return code == kInvalidCode ? zero : f;
+}
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
@@ -681,17 +708,17 @@ encode_wave_kernel(const uint32_t* __restrict__ gtable, int D, int W32,
     constexpr int W32P = 256 + 64 * NB32;    // table row length (words)
     static_assert(KH * W32P <= kPairStride, "pair-table entry overflows its stride");
     extern __shared__ uint32_t smem[];
-    uint32_t* tab2 = smem;  // [16][kPairStride]: [KH][W32P], words >= W32 zero
+    uint32_t* tab2 = smem;  // [17][kPairStride]: [KH][W32P], words >= W32 and entry 16 zero
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    uint32_t* codes = smem + 16 * kPairStride + wave * kWaveChunk;
+    uint32_t* codes = smem + (kZeroEntry + 1) * kPairStride + wave * kWaveChunk;
     const int kW = K * W32;
-    for (int i = tid; i < 16 * KH * W32P; i += blockDim.x) {
+    for (int i = tid; i < (kZeroEntry + 1) * KH * W32P; i += blockDim.x) {
         const int v = i / (KH * W32P), rem = i - v * (KH * W32P), j = rem / W32P,
                   ww = rem - j * W32P;
         tab2[v * kPairStride + rem] =
-            ww < W32 ? gtable[(v & 3) * kW + (2 * j) * W32 + ww] ^
-                           gtable[(v >> 2) * kW + (2 * j + 1) * W32 + ww]
-                     : 0u;
+            (ww < W32 && v < kZeroEntry) ? gtable[(v & 3) * kW + (2 * j) * W32 + ww] ^
+                                               gtable[(v >> 2) * kW + (2 * j + 1) * W32 + ww]
+                                         : 0u;
     }
     __syncthreads();
 
@@ -719,8 +746,13 @@ encode_wave_kernel(const uint32_t* __restrict__ gtable, int D, int W32,
             // the previous chunk's code reads are done before they are overwritten
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
-            for (int j = lane; j < cnb; j += 64)
-                codes[j] = kmer_code(packed, invalid, off + c0 + j, j, cn, K, mask2k, maskk);
+            for (int j0 = 0; j0 < cnb; j0 += 64) {  // same trip count on every lane
+                const int j = j0 + lane;
+                const uint32_t code = kmer_code(packed, invalid, off + c0 + j, j, cn, K, mask2k,
+                                                maskk);
+                if (j < cnb) codes[j] = pair_fields<KH>(code);
+                nvalid += __builtin_popcountll(__ballot(code != kInvalidCode));
+            }
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             for (int j = 0; j < cn; j += 16) {
@@ -732,7 +764,7 @@ encode_wave_kernel(const uint32_t* __restrict__ gtable, int D, int W32,
                     const uint32_t code = __builtin_amdgcn_readfirstlane(c[u]);
 #pragma unroll
                     for (int jj = 0; jj < KH; ++jj) {
-                        const uint32_t e = (code >> (4 * jj)) & 15u;
+                        const uint32_t e = (code >> (5 * jj)) & 31u;
                         const u32x4 v = pair_row4(e, a128 + 4u * jj * W32P);
                         rr[jj][0] = v.x; rr[jj][1] = v.y; rr[jj][2] = v.z; rr[jj][3] = v.w;
                         if constexpr (NB32) rr[jj][4] = pair_row(e, a32 + 4u * jj * W32P);
@@ -745,14 +777,11 @@ encode_wave_kernel(const uint32_t* __restrict__ gtable, int D, int W32,
                     if (u + 1 < 16) issue(u + 1, r[(u + 1) & 1]);
                     __builtin_amdgcn_sched_barrier(0);
                     const uint32_t(&q)[KH][NW] = r[u & 1];
-                    const bool valid = __builtin_amdgcn_readfirstlane(c[u]) != kInvalidCode;
-                    nvalid += valid ? 1 : 0;
                     uint32_t y[NW];
 #pragma unroll
                     for (int g = 0; g < NW; ++g) {
-                        uint32_t v = xor3(xor3(q[0][g], q[1][g], q[2][g]), q[3][g], q[4][g]);
-                        if constexpr (K == 12) v ^= q[5][g];
-                        y[g] = valid ? v : 0u;
+                        y[g] = xor3(xor3(q[0][g], q[1][g], q[2][g]), q[3][g], q[4][g]);
+                        if constexpr (K == 12) y[g] ^= q[5][g];
                     }
                     if ((u & 1) == 0) {
 #pragma unroll
@@ -834,7 +863,7 @@ int launch_kp(hvk_ctx* ctx, const uint32_t* packed, const uint32_t* invalid,
     int grid = n < 4 * ctx->n_cu ? n : 4 * ctx->n_cu;
     if constexpr ((K == 10 || K == 12) && P <= 16) {
         // one wave per sequence when a lane's four (or five) words cover the row
-        const size_t lds_wave = sizeof(uint32_t) * ((size_t)16 * kPairStride +
+        const size_t lds_wave = sizeof(uint32_t) * ((size_t)(kZeroEntry + 1) * kPairStride +
                                                     (size_t)kWaveWaves * kWaveChunk);
         const int nb32 = W32 <= 256 ? 0 : (W32 <= 320 ? 1 : -1);
         if (nb32 >= 0 && W32 > 128 && lds_wave + 512 <= (size_t)ctx->max_lds &&
