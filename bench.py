@@ -112,11 +112,15 @@ def cpu_rows(w, n_take, db_host, dbn, n_refs, dim, kmer, topk, gpu_idx, cpu_seco
             "topk_equal_gpu": bool(a["topk_equal_gpu"] and rows["cpu_1"]["topk_equal_gpu"])}
 
 
+POWER_SAMPLING = True  # --no-power turns it off (profiled runs: amd-smi is a script whose
+                       # interpreter start would be an exec under the profiler's preload)
+
+
 class PowerSampler:
     """Samples the board's socket power and graphics clocks (amd-smi metric)
     about once a second on a host thread while GPU work runs (kept out of
     timed regions: a query reaches the card's power controller).  Without
-    amd-smi it records nothing."""
+    amd-smi, or with --no-power, it records nothing."""
 
     def __init__(self):
         self.samples = []
@@ -124,7 +128,7 @@ class PowerSampler:
         self._t = None
 
     def __enter__(self):
-        if shutil.which("amd-smi"):
+        if POWER_SAMPLING and shutil.which("amd-smi"):
             self._t = threading.Thread(target=self._run, daemon=True)
             self._t.start()
         return self
@@ -205,6 +209,8 @@ def parse():
     ap.add_argument("--no-two-limb", action="store_true")
     ap.add_argument("--no-gemm-ref", action="store_true",
                     help="skip the library int8 GEMM reference point")
+    ap.add_argument("--no-power", action="store_true",
+                    help="no amd-smi power sampling (for runs under rocprofv3)")
     ap.add_argument("--workload", choices=["classify", "precision", "descent", "batch", "config0"],
                     default="classify",
                     help="classify: the reads/s line (1M reads x 100k refs, top-5); config0: D=8192, "
@@ -229,7 +235,9 @@ def parse():
 
 
 def main():
+    global POWER_SAMPLING
     a = parse()
+    POWER_SAMPLING = not a.no_power
     if a.gpus != 1:
         print("bench.py measures one GPU only: there is no multi-GPU path "
               "(the search would shard by reference prototypes).", file=sys.stderr)
@@ -443,7 +451,7 @@ def encode_line(enc_ms, enc_gbs, nq, lens):
     against the chip's peak (one wave64 VALU instruction per CU per clock at
     the nominal clock)."""
     import torch
-    kernel = "encode_pair_kernel<12,11>"
+    kernel = "encode_wave_kernel<12,11,1>"
     t = float(np.mean(enc_ms)) / 1000.0
     line = {"kernel": kernel, "kernel_ms": 1000.0 * t, "achieved_gbs": enc_gbs,
             "frac_hbm": enc_gbs / HBM_PEAK_GBS, "roofline": None}
@@ -458,6 +466,8 @@ def encode_line(enc_ms, enc_gbs, nq, lens):
             "frac": achieved / peak,
             "valu_insts_per_kmer_word": c["SQ_INSTS_VALU"] * 64 / (kmers * ((DIM + 31) // 32)),
             "lds_insts_per_valu": c.get("SQ_INSTS_LDS", 0.0) / c["SQ_INSTS_VALU"],
+            "lds_busy": (c["SQ_LDS_IDX_ACTIVE"] / n_cu / (c["GRBM_GUI_ACTIVE"] / 8)
+                         if "SQ_LDS_IDX_ACTIVE" in c and "GRBM_GUI_ACTIVE" in c else None),
             "source": "rocprofv3 --pmc SQ_INSTS_VALU et al., profiles/encode_config1_pmc.md"}
     return line
 
