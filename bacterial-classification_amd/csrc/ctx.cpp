@@ -169,6 +169,10 @@ int hvk_ctx_set_tuning(hvk_ctx* ctx, int param, int value) {
             if (value < 0 || value > 1) return HVK_ERR_ARG;
             ctx->encode_legacy = value == 1;
             return HVK_OK;
+        case HVK_TUNE_DESCENT:
+            if (value < 0 || value > 1) return HVK_ERR_ARG;
+            ctx->tune_desc_i16 = value == 1;
+            return HVK_OK;
         default:
             return HVK_ERR_ARG;
     }

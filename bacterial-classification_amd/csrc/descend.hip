@@ -3,22 +3,29 @@
 // category, scored by the reference's cosine (src/HV.cpp:219-242) from exact
 // int64 dots.
 //
-// One level is six launches on the caller's stream, with no host round trip:
-//   1. bucket count: one key per live (read, beam entry) = its category
+// One level is seven launches on the caller's stream, with no host round trip:
+//   1. bucket count: one key per live (read, beam entry) = its category, on
+//      one of two paths: int8 (the read's counts all fit int8 and every
+//      candidate of the category splits into two int8 digits) or int16
 //   2. bucket scan (tile sums, one-workgroup scan of the sums, tile scans)
-//   3. scatter: (read, entry) pairs grouped by category
-//   4. score: every pair against the candidate list of its category.  A
-//      wave takes 4 consecutive pairs of the grouped list (mostly one
-//      category) and splits K across its lanes, so every load reads 1 KB of
-//      one row; each child slab is loaded once for all of the wave's pairs
-//      of that category.  v_dot2_i32_i16 multiplies int16 counts by the
-//      int16 limbs (p = 65536 hi + lo) into int32 partial sums, widened to
-//      int64 every E elements per lane, E chosen so no partial can overflow;
-//      the lane sums are reduced across the wave.
-//   5. select: per read, gather its candidates in identify's order (beam
+//   3. scatter: (read, entry) pairs grouped by path, then category
+//   4. score, int8 path: v_dot4_i32_i8 on the read's int8 row and the
+//      candidates' two digit planes (p = d0 + 256 d1), eight pairs per wave
+//   5. score, int16 path: every other pair against the candidate list of its
+//      category.  A wave takes 4 consecutive pairs of the grouped list
+//      (mostly one category) and splits K across its lanes, so every load
+//      reads 1 KB of one row; each child slab is loaded once for all of the
+//      wave's pairs of that category.  v_dot2_i32_i16 multiplies int16
+//      counts by the int16 limbs (p = 65536 hi + lo) into int32 partial
+//      sums, widened to int64 every E elements per lane, E chosen so no
+//      partial can overflow; the lane sums are reduced across the wave.
+//   6. select: per read, gather its candidates in identify's order (beam
 //      order, then each entry's candidate list), sort them exactly as
 //      std::sort does (lsort.h), then exit or keep the beam.
-// A final launch writes the surviving reads' best paths.
+// Level 0, where every read scores the roots, is one dense matrix-core
+// product that also writes the int8 read rows.  Both paths compute the same
+// exact integer dots, so the path taken never changes a score.  A final
+// launch writes the surviving reads' best paths.
 #include <algorithm>
 #include <vector>
 
@@ -106,18 +113,30 @@ __global__ void desc_init(Beams b, int n, int B) {
     b.cat[(int64_t)r * B] = -1;
 }
 
-// key of an entry: its category, n_cat for the root
-__global__ void desc_hist(Beams b, int n, int B, int n_cat, int32_t* hist) {
+// key of an entry: its category, n_cat for the root.  The bucket of a pair is
+// (path, key, copy), path-major: pairs whose read has an int8 row (qmode 0)
+// and whose candidates all have digit planes (cflag) come first, scored by
+// desc_score8; the others follow, scored by desc_score from the int16 rows.
+// Without the int8 rows (qmode null) every pair takes path 1.
+__device__ __forceinline__ int pair_bucket(int key, int r, int n_cat, const uint8_t* qmode,
+                                           const uint8_t* cflag, int copy) {
+    const int path = (qmode && qmode[r] == 0 && cflag[key]) ? 0 : 1;
+    return (path * (n_cat + 1) + key) * kDescCopies + copy;
+}
+
+__global__ void desc_hist(Beams b, int n, int B, int n_cat, const uint8_t* __restrict__ qmode,
+                          const uint8_t* __restrict__ cflag, int32_t* hist) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     const int nb = r < n ? b.nb[r] : 0;
     for (int j = 0; j < B; ++j) {
         const bool act = j < nb;
-        int key = 0;
+        int bucket = 0;
         if (act) {
             const int c = b.cat[(int64_t)r * B + j];
-            key = c < 0 ? n_cat : c;
+            bucket = pair_bucket(c < 0 ? n_cat : c, r, n_cat, qmode, cflag,
+                                 (int)(blockIdx.x % kDescCopies));
         }
-        (void)wave_agg_add(hist, key * kDescCopies + (int)(blockIdx.x % kDescCopies), act);
+        (void)wave_agg_add(hist, bucket, act);
     }
 }
 
@@ -200,18 +219,20 @@ desc_scan_apply(const int32_t* __restrict__ hist, int n_keys, const int32_t* __r
     }
 }
 
-__global__ void desc_scatter(Beams b, int n, int B, int n_cat, int32_t* cursor, int32_t* pairs,
+__global__ void desc_scatter(Beams b, int n, int B, int n_cat, const uint8_t* __restrict__ qmode,
+                             const uint8_t* __restrict__ cflag, int32_t* cursor, int32_t* pairs,
                              int32_t* pkey) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     const int nb = r < n ? b.nb[r] : 0;
     for (int j = 0; j < B; ++j) {
         const bool act = j < nb;
-        int key = 0;
+        int key = 0, bucket = 0;
         if (act) {
             const int c = b.cat[(int64_t)r * B + j];
             key = c < 0 ? n_cat : c;
+            bucket = pair_bucket(key, r, n_cat, qmode, cflag, (int)(blockIdx.x % kDescCopies));
         }
-        const int pos = wave_agg_add(cursor, key * kDescCopies + (int)(blockIdx.x % kDescCopies), act);
+        const int pos = wave_agg_add(cursor, bucket, act);
         if (act) {
             pairs[pos] = r * B + j;
             pkey[pos] = key;
@@ -219,12 +240,92 @@ __global__ void desc_scatter(Beams b, int n, int B, int n_cat, int32_t* cursor, 
     }
 }
 
+// Read rows as int8: q8[r] = count[r] where every count is in [-128, 127]
+// (then the row is exact in int8), qmode[r] = 0; else qmode[r] = 1 and the
+// int16 row is used.  One wave per read; columns [dim, K) are zero in every
+// valid row.
+__global__ void desc_q8(const int16_t* __restrict__ q, int64_t qld, int n, int64_t K,
+                        int8_t* __restrict__ q8, uint8_t* __restrict__ qmode) {
+    const int r = blockIdx.x * (kDescThreads / 64) + (threadIdx.x >> 6);
+    if (r >= n) return;  // uniform per wave
+    const int lane = threadIdx.x & 63;
+    const int16_t* row = q + (int64_t)r * qld;
+    int8_t* out = q8 + (int64_t)r * K;
+    bool bad = false;
+    for (int64_t k = 16 * lane; k < K; k += 1024) {
+        const int4 a = *reinterpret_cast<const int4*>(row + k);
+        const int4 b = *reinterpret_cast<const int4*>(row + k + 8);
+        const int w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+        uint32_t o[4];
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+            const int lo0 = (int)(int16_t)w[2 * d], hi0 = w[2 * d] >> 16;
+            const int lo1 = (int)(int16_t)w[2 * d + 1], hi1 = w[2 * d + 1] >> 16;
+            bad |= ((uint32_t)(lo0 + 128) | (uint32_t)(hi0 + 128) | (uint32_t)(lo1 + 128) |
+                    (uint32_t)(hi1 + 128)) > 255u;
+            o[d] = (uint32_t)(lo0 & 0xFF) | ((uint32_t)(hi0 & 0xFF) << 8) |
+                   ((uint32_t)(lo1 & 0xFF) << 16) | ((uint32_t)(hi1 & 0xFF) << 24);
+        }
+        *reinterpret_cast<uint4*>(out + k) = make_uint4(o[0], o[1], o[2], o[3]);
+    }
+    const bool any_bad = __ballot(bad) != 0;  // every lane of the wave votes
+    if (lane == 0) qmode[r] = any_bad ? 1 : 0;
+}
+
+// Prototype rows as two signed base-256 int8 digits, p = d0 + 256 d1, where
+// every |p| <= 32639 (dflag[c] = 1), for the int8 path of desc_score; rows
+// with a larger entry keep dflag[c] = 0 and the int16 limbs.  One wave per
+// category.
+__global__ void desc_digits(TreeDev t, int64_t K, int8_t* __restrict__ d0, int8_t* __restrict__ d1,
+                            uint8_t* __restrict__ dflag) {
+    const int c = blockIdx.x * (kDescThreads / 64) + (threadIdx.x >> 6);
+    if (c >= t.n_cat) return;  // uniform per wave
+    const int lane = threadIdx.x & 63;
+    const int16_t* lo = t.lo + (int64_t)c * t.ld;
+    const int16_t* hi = t.hi + (int64_t)c * t.ld;
+    int8_t* o0 = d0 + (int64_t)c * K;
+    int8_t* o1 = d1 + (int64_t)c * K;
+    bool ok = true;
+    for (int64_t k = 8 * lane; k < K; k += 512) {
+        const int4 a = *reinterpret_cast<const int4*>(lo + k);
+        const int4 b = *reinterpret_cast<const int4*>(hi + k);
+        const int la[4] = {a.x, a.y, a.z, a.w}, hb[4] = {b.x, b.y, b.z, b.w};
+        uint32_t x0[2] = {0, 0}, x1[2] = {0, 0};
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int l = (int)(int16_t)(la[e >> 1] >> (16 * (e & 1)));
+            const int h = (int)(int16_t)(hb[e >> 1] >> (16 * (e & 1)));
+            const int64_t p = (int64_t)l + 65536 * (int64_t)h;
+            ok = ok && p >= -32639 && p <= 32639;
+            const int v0 = (int)(int8_t)(p & 0xFF);
+            const int v1 = (int)((p - v0) >> 8);
+            x0[e >> 2] |= (uint32_t)(v0 & 0xFF) << (8 * (e & 3));
+            x1[e >> 2] |= (uint32_t)(v1 & 0xFF) << (8 * (e & 3));
+        }
+        *reinterpret_cast<uint2*>(o0 + k) = make_uint2(x0[0], x0[1]);
+        *reinterpret_cast<uint2*>(o1 + k) = make_uint2(x1[0], x1[1]);
+    }
+    const bool all = __ballot(!ok) == 0;
+    if (lane == 0) dflag[c] = all ? 1 : 0;
+}
+
+// The int8 rows of desc_score8: read rows q8 (qmode 0) and prototype digit
+// planes d0, d1 (dflag 1), row stride K; cflag per candidate list.
+struct DescI8 {
+    int8_t* q8;
+    uint8_t* qmode;
+    int8_t* d0;
+    int8_t* d1;
+    uint8_t* dflag;
+    uint8_t* cflag;
+};
+
 __device__ __forceinline__ int32_t sdot2(int32_t a, int32_t b, int32_t c) {
     return __builtin_amdgcn_sdot2(__builtin_bit_cast(short2_t, a), __builtin_bit_cast(short2_t, b),
                                   c, false);
 }
 
-// Scores pairs [0, *total) of the grouped list.  E: elements per lane
+// Scores pairs [*begin, *total) of the grouped list.  E: elements per lane
 // between widenings of the int32 partial sums (8 or 2; a 32-element chunk
 // would keep 2*P*G more partials live and halve the waves per SIMD).
 //
@@ -251,11 +352,12 @@ template <int E, int GW>
 __global__ void __launch_bounds__(kDescThreads, GW <= kDescGSmall ? 5 : 4)
 desc_score(TreeDev t, const int16_t* __restrict__ q, int64_t qld, const int64_t* __restrict__ qn,
            int B, int64_t K, const int32_t* __restrict__ pairs, const int32_t* __restrict__ pkey,
-           const int32_t* __restrict__ total, int Fmax, float* __restrict__ sc) {
+           const int32_t* __restrict__ begin, const int32_t* __restrict__ total, int Fmax,
+           float* __restrict__ sc) {
     static_assert(E == 8 || E == 2, "partial-sum chunk");
     const int n_pairs = *total;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int base = (blockIdx.x * (kDescThreads / 64) + wave) * kDescP;
+    const int base = *begin + (blockIdx.x * (kDescThreads / 64) + wave) * kDescP;
     if (base >= n_pairs) return;  // uniform per wave
     const int lane = lane_id();
     int key[kDescP], pr[kDescP];
@@ -309,7 +411,7 @@ desc_score(TreeDev t, const int16_t* __restrict__ q, int64_t qld, const int64_t*
                 for (int g = 0; g < GW; ++g) acc[p][g] = 0;
             auto pass = [&](auto HI) {
                 constexpr bool kHi = decltype(HI)::value;
-                for (int64_t k = 8 * lane; k < K; k += kDescStep) {
+                for (uint32_t k = 8 * lane; k < (uint32_t)K; k += kDescStep) {
                     int4 qv[kDescP];
 #pragma unroll
                     for (int p = 0; p < kDescP; ++p)
@@ -362,6 +464,144 @@ desc_score(TreeDev t, const int16_t* __restrict__ q, int64_t qld, const int64_t*
     }
 }
 
+// cflag[key] = 1 when every candidate of key (the children of category key,
+// the roots for key n_cat) has two digit planes.
+__global__ void desc_cflag(TreeDev t, const uint8_t* __restrict__ dflag, uint8_t* __restrict__ cflag) {
+    const int key = blockIdx.x * blockDim.x + threadIdx.x;
+    if (key > t.n_cat) return;
+    const int32_t* list;
+    int F;
+    span_of(t, key == t.n_cat ? -1 : key, list, F);
+    bool ok = true;
+    for (int i = 0; i < F; ++i) ok = ok && dflag[list[i]] != 0;
+    cflag[key] = ok ? 1 : 0;
+}
+
+// Scores pairs [0, *n_i8) of the grouped list: reads with int8 rows against
+// candidates with two int8 digit planes.  One wave takes P consecutive pairs
+// (mostly one category) and walks their candidates GW at a time; lane l holds
+// elements [4 NW l, 4 NW l + 4 NW) of every 256 NW-element step, so each load
+// reads 256 NW bytes of one row.  Each lane sums q d0 + 256 q d1 (v_dot4 on
+// int8 operands) in one int32 per (pair, candidate): with |q| <= 128 and
+// |p| <= 32639, at most K / 64 * 128 * 32639 < 2^31 for K <= 32512 (checked by
+// the host), and every partial sum of a step stays within 2^31 as well.
+// Eight pairs per wave and 8-byte lane slices: one candidate slab serves eight
+// read rows, and the 40 accumulators (G = 5), 16 read and 20 candidate
+// registers fit four waves per SIMD with every load of a step in flight
+// (measured against four pairs per wave, 16-byte slices and a loop that skips
+// the slots past the list end: 4-8% slower or equal).
+constexpr int kDescP8 = 8;
+
+template <int N>
+struct IVec {
+    typedef int32_t type __attribute__((ext_vector_type(N)));
+};
+
+template <int P, int NW, int GW, int OCC>
+__global__ void __launch_bounds__(kDescThreads, OCC)
+desc_score8(TreeDev t, const int8_t* __restrict__ q8, const int64_t* __restrict__ qn, int B,
+            int64_t K, const int8_t* __restrict__ d0, const int8_t* __restrict__ d1,
+            const int32_t* __restrict__ pairs, const int32_t* __restrict__ pkey,
+            const int32_t* __restrict__ n_i8, int Fmax, float* __restrict__ sc) {
+    typedef typename IVec<NW>::type vec;
+    const int n_pairs = *n_i8;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int base = (blockIdx.x * (kDescThreads / 64) + wave) * P;
+    if (base >= n_pairs) return;  // uniform per wave
+    const int lane = lane_id();
+    int key[P], pr[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        const bool ok = base + p < n_pairs;
+        key[p] = ok ? pkey[base + p] : -1;
+        pr[p] = ok ? pairs[base + p] : 0;
+    }
+    uint32_t todo = 0;
+#pragma unroll
+    for (int p = 0; p < P; ++p) todo |= (key[p] >= 0 ? 1u : 0u) << p;
+    while (todo) {
+        const int pk = key[__builtin_ctz(todo)];
+        uint32_t mine = 0;
+#pragma unroll
+        for (int p = 0; p < P; ++p) mine |= (key[p] == pk ? 1u : 0u) << p;
+        todo &= ~mine;
+        const int32_t* list;
+        int F;
+        span_of(t, pk == t.n_cat ? -1 : pk, list, F);
+        // every load of a step is issued unconditionally, so that one wait
+        // covers them all: a pair of another category reads the row of this
+        // pass's first pair (same cache lines), a slot past the list end
+        // reads the last candidate, and their sums are dropped
+        int rfirst = 0;
+#pragma unroll
+        for (int p = P - 1; p >= 0; --p)
+            if ((mine >> p) & 1) rfirst = pr[p];
+        const int8_t* qrow[P];
+#pragma unroll
+        for (int p = 0; p < P; ++p)
+            qrow[p] = q8 + (int64_t)((((mine >> p) & 1) ? pr[p] : rfirst) / B) * K;
+        for (int g0 = 0; g0 < F; g0 += GW) {
+            const int G = min(GW, F - g0);
+            const int8_t* u_row[GW];
+            const int8_t* v_row[GW];
+#pragma unroll
+            for (int g = 0; g < GW; ++g) {
+                const int c = list[g0 + min(g, G - 1)];
+                u_row[g] = d0 + (int64_t)c * K;
+                v_row[g] = d1 + (int64_t)c * K;
+            }
+            int32_t a[P][GW];
+#pragma unroll
+            for (int p = 0; p < P; ++p)
+#pragma unroll
+                for (int g = 0; g < GW; ++g) a[p][g] = 0;
+            // 32-bit lane offsets: each load is a uniform row base plus one
+            // VGPR (K < 2^31 bytes)
+#pragma nounroll
+            for (uint32_t k = 4 * NW * lane; k < (uint32_t)K; k += 256 * NW) {
+                vec qv[P];
+#pragma unroll
+                for (int p = 0; p < P; ++p) qv[p] = *reinterpret_cast<const vec*>(qrow[p] + k);
+                vec uv[GW], vv[GW];
+#pragma unroll
+                for (int g = 0; g < GW; ++g) {
+                    uv[g] = *reinterpret_cast<const vec*>(u_row[g] + k);
+                    vv[g] = *reinterpret_cast<const vec*>(v_row[g] + k);
+                }
+#pragma unroll
+                for (int g = 0; g < GW; ++g) {
+                    if (g >= G) continue;  // uniform
+#pragma unroll
+                    for (int p = 0; p < P; ++p) {
+                        int32_t t1 = 0;
+#pragma unroll
+                        for (int w = 0; w < NW; ++w)
+                            t1 = __builtin_amdgcn_sdot4(qv[p][w], vv[g][w], t1, false);
+                        int32_t s0 = a[p][g] + (t1 << 8);
+#pragma unroll
+                        for (int w = 0; w < NW; ++w)
+                            s0 = __builtin_amdgcn_sdot4(qv[p][w], uv[g][w], s0, false);
+                        a[p][g] = s0;
+                    }
+                }
+            }
+#pragma unroll
+            for (int p = 0; p < P; ++p)
+#pragma unroll
+                for (int g = 0; g < GW; ++g) {
+                    if (g < G && ((mine >> p) & 1)) {
+                        const int64_t dot = wave_sum_i64((int64_t)a[p][g]);
+                        if (lane == 0) {
+                            const int r = pr[p] / B;
+                            const int c = list[g0 + g];
+                            sc[(int64_t)pr[p] * Fmax + g0 + g] = ref_score(dot, qn[r], t.sqnorm[c]);
+                        }
+                    }
+                }
+        }
+    }
+}
+
 // Level 0 as a limb GEMM on the matrix cores.  At level 0 every read scores
 // the same candidate list (the roots), so the scores are a dense product
 // reads x roots x D.  Operands are split into signed base-256 digits: a read
@@ -378,6 +618,7 @@ typedef int v4i_t __attribute__((ext_vector_type(4)));
 typedef int v16i_t __attribute__((ext_vector_type(16)));
 constexpr int kDenseN = 64;     // roots (two 32-wide blocks)
 constexpr int kDenseRow = 48;   // LDS bytes per (digit, root) row: 32 + 16 pad, conflict-free
+constexpr int kDenseStage = 144;  // LDS bytes per staged int8 read row: 128 + 16 pad
 
 __device__ __forceinline__ uint32_t pack4(int a, int b, int c, int d) {
     return (uint32_t)(a & 0xFF) | ((uint32_t)(b & 0xFF) << 8) | ((uint32_t)(c & 0xFF) << 16) |
@@ -389,9 +630,13 @@ __device__ __forceinline__ uint32_t pack4(int a, int b, int c, int d) {
 template <int ND>
 __global__ void __launch_bounds__(256, 2)
 desc_dense(TreeDev t, const int16_t* __restrict__ q, int64_t qld, const int64_t* __restrict__ qn,
-           int n, int B, int64_t K, int Fmax, float* __restrict__ sc) {
+           int n, int B, int64_t K, int Fmax, float* __restrict__ sc, int8_t* __restrict__ q8,
+           uint8_t* __restrict__ qmode) {
     static_assert(ND >= 2 && ND <= 4, "prototype digits");
     __shared__ __attribute__((aligned(16))) int8_t bl[ND][kDenseN][kDenseRow];
+    // the int8 rows pass through LDS, 128 K bytes of the wave's 32 reads at a
+    // time, so that each store writes whole 128-byte lines (8 rows x 128 B)
+    __shared__ __attribute__((aligned(16))) int8_t qst[4][32][kDenseStage];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int r0 = (blockIdx.x * 4 + wave) * 32;
     const int i = lane & 31, kg = lane >> 5;
@@ -406,6 +651,7 @@ desc_dense(TreeDev t, const int16_t* __restrict__ q, int64_t qld, const int64_t*
     for (int w = 0; w < ND + 1; ++w)
 #pragma unroll
         for (int nb = 0; nb < 2; ++nb) acc[w][nb] = v16i_t{};
+    int qbad = 0;  // nonzero when some q of this lane's part of the row is outside int8
     for (int64_t k0 = 0; k0 < K; k0 += 32) {
         __syncthreads();  // the previous step's digits are consumed
         {
@@ -456,6 +702,20 @@ desc_dense(TreeDev t, const int16_t* __restrict__ q, int64_t qld, const int64_t*
             a[0][c] = (int)pack4(q0[4 * c], q0[4 * c + 1], q0[4 * c + 2], q0[4 * c + 3]);
             a[1][c] = (int)pack4(q1[4 * c], q1[4 * c + 1], q1[4 * c + 2], q1[4 * c + 3]);
         }
+        if (q8) {  // the int8 rows of the later levels: the low digit, exact where q1 is 0
+            qbad |= a[1][0] | a[1][1] | a[1][2] | a[1][3];
+            const int s4 = (int)(k0 >> 5) & 3;  // K is a multiple of 128
+            *reinterpret_cast<v4i_t*>(&qst[wave][i][32 * s4 + 16 * kg]) = a[0];
+            if (s4 == 3) {  // wave-local: this wave's own LDS rows, in program order
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int row = 8 * j + (lane >> 3), col = 16 * (lane & 7);
+                    const v4i_t v = *reinterpret_cast<const v4i_t*>(&qst[wave][row][col]);
+                    if (r0 + row < n)
+                        *reinterpret_cast<v4i_t*>(q8 + (int64_t)(r0 + row) * K + (k0 - 96) + col) = v;
+                }
+            }
+        }
         __syncthreads();  // the step's digits are staged
 #pragma unroll
         for (int nb = 0; nb < 2; ++nb)
@@ -467,6 +727,10 @@ desc_dense(TreeDev t, const int16_t* __restrict__ q, int64_t qld, const int64_t*
                     acc[qa_i + g][nb] =
                         __builtin_amdgcn_mfma_i32_32x32x32_i8(a[qa_i], b, acc[qa_i + g][nb], 0, 0, 0);
             }
+    }
+    if (q8) {
+        qbad |= __shfl_xor(qbad, 32);  // lanes i and i + 32 hold one read
+        if (kg == 0 && r0 + i < n) qmode[r0 + i] = qbad ? 1 : 0;
     }
     // lane l holds column l % 32, rows 8 (e / 4) + 4 (l / 32) + e % 4 of each block
 #pragma unroll
@@ -742,8 +1006,8 @@ int desc_prepare(hvk_ctx* ctx, const hvk_tree* tree, int q_maxabs, int beam, int
 }
 
 // The launches of a descent on a prepared plan: the workspace (grown here
-// when it is too small) and six launches per level on st, with no host
-// synchronisation.  A stream capture of it records launches only when the
+// when it is too small) and a fixed sequence of launches per level on st,
+// with no host synchronisation.  A stream capture of it records launches only when the
 // workspace already holds n_q reads.
 int desc_launch(hvk_ctx* ctx, const DescPlan& pl, const int16_t* d_q_counts,
                 const int64_t* d_q_sqnorm, int n_q, int64_t q_ld, float threshold,
@@ -760,12 +1024,20 @@ int desc_launch(hvk_ctx* ctx, const DescPlan& pl, const int16_t* d_q_counts,
     const size_t beam_bytes = align_up(n * 4) + align_up(n * B * 4) + 2 * align_up(n * B * L * 4);
     const size_t sc_bytes = align_up((size_t)n * B * Fmax * 4);
     const size_t pair_bytes = 2 * align_up((size_t)n * B * 4);
-    const int n_keys = (n_cat + 1) * kDescCopies;  // counter slots
+    const int n_keys = 2 * (n_cat + 1) * kDescCopies;  // counter slots: (path, key, copy)
     const int n_scan = (n_keys + kScanTile - 1) / kScanTile;  // scan tiles
-    const size_t key_bytes = 2 * align_up(((size_t)n_cat + 1) * kDescCopies * 4) + 256 +
-                             align_up((size_t)n_scan * 4);
+    const size_t key_bytes = 2 * align_up((size_t)n_keys * 4) + 256 + align_up((size_t)n_scan * 4);
     const size_t sort_bytes = 2 * align_up((size_t)n * maxN * 4);
-    if (int e = ctx->ws_desc.ensure(2 * beam_bytes + sc_bytes + pair_bytes + key_bytes + sort_bytes))
+    // the int8 rows of the scoring levels: reads (K bytes each) and the
+    // prototypes' two digit planes, with their flags
+    const int64_t K = ctx->ld;
+    const bool use_i8 = !ctx->tune_desc_i16 && L > (dense ? 1 : 0) && K <= 32512;
+    const size_t i8_bytes = use_i8 ? align_up((size_t)n * K) + align_up((size_t)n) +
+                                         2 * align_up((size_t)n_cat * K) + align_up((size_t)n_cat) +
+                                         align_up((size_t)n_cat + 1)
+                                   : 0;
+    if (int e = ctx->ws_desc.ensure(2 * beam_bytes + sc_bytes + pair_bytes + key_bytes + sort_bytes +
+                                    i8_bytes))
         return e;
     char* w = (char*)ctx->ws_desc.ptr;
     auto take = [&](size_t bytes) { char* p = w; w += align_up(bytes); return p; };
@@ -779,19 +1051,29 @@ int desc_launch(hvk_ctx* ctx, const DescPlan& pl, const int16_t* d_q_counts,
     float* sc = (float*)take((size_t)n * B * Fmax * 4);
     int32_t* pairs = (int32_t*)take((size_t)n * B * 4);
     int32_t* pkey = (int32_t*)take((size_t)n * B * 4);
-    int32_t* hist = (int32_t*)take(((size_t)n_cat + 1) * kDescCopies * 4);
-    int32_t* cursor = (int32_t*)take(((size_t)n_cat + 1) * kDescCopies * 4);
+    int32_t* hist = (int32_t*)take((size_t)n_keys * 4);
+    int32_t* cursor = (int32_t*)take((size_t)n_keys * 4);
     int32_t* total = (int32_t*)take(256);
     int32_t* scan_part = (int32_t*)take((size_t)n_scan * 4);
     float* ws_s = (float*)take((size_t)n * maxN * 4);
     int32_t* ws_c = (int32_t*)take((size_t)n * maxN * 4);
+    DescI8 i8{};
+    if (use_i8) {
+        i8.q8 = (int8_t*)take((size_t)n * K);
+        i8.qmode = (uint8_t*)take((size_t)n);
+        i8.d0 = (int8_t*)take((size_t)n_cat * K);
+        i8.d1 = (int8_t*)take((size_t)n_cat * K);
+        i8.dflag = (uint8_t*)take((size_t)n_cat);
+        i8.cflag = (uint8_t*)take((size_t)n_cat + 1);
+    }
 
     const DescOut out{d_path, d_path_score, d_path_len, d_exit_level};
     const int rb = (int)((n + kDescThreads - 1) / kDescThreads);
     const int pb = (int)((n * B + kDescP * (kDescThreads / 64) - 1) / (kDescP * (kDescThreads / 64)));
+    const int pb8 =
+        (int)((n * B + kDescP8 * (kDescThreads / 64) - 1) / (kDescP8 * (kDescThreads / 64)));
     // columns [dim, ctx->ld) of every row are zero, and any valid stride is at
-    // least ctx->ld: the dots run over the context's ld, whatever the strides
-    const int64_t K = ctx->ld;
+    // least ctx->ld: the dots run over the context's ld (K), whatever the strides
 
     // candidate lists of up to kSelMaxLds entries are sorted in LDS
     constexpr int kSelThreads = 128, kSelMaxLds = 64;
@@ -800,6 +1082,20 @@ int desc_launch(hvk_ctx* ctx, const DescPlan& pl, const int16_t* d_q_counts,
     // level 0 on the matrix cores when its operands fit the digit split
     timing_mark(ctx, 1, false, st);
     hipLaunchKernelGGL(desc_init, dim3(rb), dim3(kDescThreads), 0, st, bm[0], n_q, B);
+    if (use_i8) {
+        const int wpb = kDescThreads / 64;
+        if (!dense)  // else level 0 writes them
+            hipLaunchKernelGGL(desc_q8, dim3((unsigned)((n + wpb - 1) / wpb)), dim3(kDescThreads),
+                               0, st, d_q_counts, q_ld, n_q, K, i8.q8, i8.qmode);
+        if (n_cat > 0)
+            hipLaunchKernelGGL(desc_digits, dim3((unsigned)((n_cat + wpb - 1) / wpb)),
+                               dim3(kDescThreads), 0, st, t, K, i8.d0, i8.d1, i8.dflag);
+        hipLaunchKernelGGL(desc_cflag, dim3((unsigned)((n_cat + kDescThreads) / kDescThreads)),
+                           dim3(kDescThreads), 0, st, t, i8.dflag, i8.cflag);
+    }
+    // after the scatter, the cursor of the last path-0 bucket holds the
+    // number of path-0 pairs: the first pair of desc_score
+    const int32_t* n_i8 = cursor + (n_cat + 1) * kDescCopies - 1;
     HVK_HIP(hipGetLastError());
     int cur = 0;
     for (int lvl = 0; lvl < L; ++lvl) {
@@ -808,25 +1104,33 @@ int desc_launch(hvk_ctx* ctx, const DescPlan& pl, const int16_t* d_q_counts,
                            : pl.max_abs_proto <= 8355711 ? desc_dense<3>
                                                          : desc_dense<4>;
             hipLaunchKernelGGL(dense_k, dim3((n_q + 127) / 128), dim3(256), 0, st, t, d_q_counts,
-                               q_ld, d_q_sqnorm, n_q, B, K, Fmax, sc);
+                               q_ld, d_q_sqnorm, n_q, B, K, Fmax, sc, i8.q8, i8.qmode);
         } else {
             hipLaunchKernelGGL(desc_fill, dim3((n_keys + kDescThreads - 1) / kDescThreads),
                                dim3(kDescThreads), 0, st, hist, (int64_t)n_keys, 0);
             hipLaunchKernelGGL(desc_hist, dim3(rb), dim3(kDescThreads), 0, st, bm[cur], n_q, B,
-                               n_cat, hist);
+                               n_cat, i8.qmode, i8.cflag, hist);
             hipLaunchKernelGGL(desc_scan_sums, dim3(n_scan), dim3(1024), 0, st, hist, n_keys,
                                scan_part);
             hipLaunchKernelGGL(desc_scan_top, dim3(1), dim3(1024), 0, st, scan_part, n_scan, total);
             hipLaunchKernelGGL(desc_scan_apply, dim3(n_scan), dim3(1024), 0, st, hist, n_keys,
                                scan_part, cursor);
             hipLaunchKernelGGL(desc_scatter, dim3(rb), dim3(kDescThreads), 0, st, bm[cur], n_q, B,
-                               n_cat, cursor, pairs, pkey);
-            auto score = [&](auto kernel) {
-                hipLaunchKernelGGL(kernel, dim3(pb), dim3(kDescThreads), 0, st, t, d_q_counts, q_ld,
-                                   d_q_sqnorm, B, K, pairs, pkey, total, Fmax, sc);
-            };
+                               n_cat, i8.qmode, i8.cflag, cursor, pairs, pkey);
             // the longest candidate list of this level: the roots at level 0
             const bool narrow = (lvl == 0 ? pl.n_roots : max_span) <= kDescGSmall;
+            if (use_i8) {
+                auto score8 = [&](auto kernel) {
+                    hipLaunchKernelGGL(kernel, dim3(pb8), dim3(kDescThreads), 0, st, t, i8.q8,
+                                       d_q_sqnorm, B, K, i8.d0, i8.d1, pairs, pkey, n_i8, Fmax, sc);
+                };
+                if (narrow) score8(desc_score8<kDescP8, 2, kDescGSmall, 4>);
+                else score8(desc_score8<kDescP8, 2, kDescG, 2>);
+            }
+            auto score = [&](auto kernel) {
+                hipLaunchKernelGGL(kernel, dim3(pb), dim3(kDescThreads), 0, st, t, d_q_counts, q_ld,
+                                   d_q_sqnorm, B, K, pairs, pkey, n_i8, total, Fmax, sc);
+            };
             if (E == 8) {
                 if (narrow) score(desc_score<8, kDescGSmall>);
                 else score(desc_score<8, kDescG>);

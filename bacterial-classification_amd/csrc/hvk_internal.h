@@ -73,6 +73,7 @@ struct hvk_ctx {
     int tune_select_kib = 0;  // large-k selection workspace budget, 0 = 1 GiB
     int tune_shape = 0;       // tile shape of the bounded search (8-entry lists)
     bool encode_legacy = false;  // workgroup-per-sequence encoders only
+    bool tune_desc_i16 = false;  // descent: int16 read rows only
 };
 
 // Kernel launchers (defined in the .hip files).

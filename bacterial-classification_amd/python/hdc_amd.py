@@ -45,6 +45,7 @@ TUNE_SPLITS = 1
 TUNE_SELECT_KIB = 2
 TUNE_SHAPE = 3
 TUNE_ENCODER = 4
+TUNE_DESCENT = 5
 
 
 class HvkError(RuntimeError):

@@ -82,11 +82,17 @@ int hvk_ctx_last_timing(hvk_ctx* ctx, float* encode_ms, float* search_ms);
  * 2 = 256 x 128, 3 = 128 x 512, 4 = 512 x 128, 5 = 256 x 256 on a 4 x 2
  * wave grid.
  * HVK_TUNE_ENCODER: 0 = automatic (the default: one wave per sequence where
- * it applies), 1 = the workgroup-per-sequence encoders only. */
+ * it applies), 1 = the workgroup-per-sequence encoders only.
+ * HVK_TUNE_DESCENT: 0 = hvk_descend and hvk_batch_descend score a read whose
+ * every count fits int8 from an int8 copy of its row, against candidates
+ * whose every |p| <= 32639 split into two int8 digits (the default); other
+ * pairs use the int16 rows.  1 = int16 rows only.  The scores are the same
+ * exact dots either way. */
 #define HVK_TUNE_SPLITS 1
 #define HVK_TUNE_SELECT_KIB 2
 #define HVK_TUNE_SHAPE 3
 #define HVK_TUNE_ENCODER 4
+#define HVK_TUNE_DESCENT 5
 int hvk_ctx_set_tuning(hvk_ctx* ctx, int param, int value);
 
 /* ---- packing (host) -----------------------------------------------------

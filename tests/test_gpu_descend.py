@@ -155,6 +155,30 @@ def test_descend_matches_oracle(synth, beam, thr):
         assert (ex == 0).sum() > n // 2
 
 
+def test_descend_int8_and_int16_rows_match_oracle(synth):
+    """By default, reads whose every count fits int8 are scored from int8 rows
+    against two int8 digit planes of the prototypes, the others from their
+    int16 rows; HVK_TUNE_DESCENT = 1 scores every read from int16 rows.  The
+    synthetic reads (150-1,600 nt) hold both kinds, and both settings give
+    the oracle's descent, with the int8 rows written by the dense level 0
+    and by the conversion kernel of the vector level 0."""
+    import hdc_amd as H
+    ctx, tax, tree, q, proto, qc, truth = synth
+    mx = np.abs(qc.astype(np.int32)).max(axis=1)
+    assert (mx <= 127).any() and (mx > 128).any()
+    try:
+        for dense in (True, False):
+            tree.use_dense = dense
+            for mode in (1, 0):
+                ctx.set_tuning(H.TUNE_DESCENT, mode)
+                print(f"descend synthetic int8 rows mode={mode} dense_level0={dense}", flush=True)
+                got, ref = _run_both(synth, 3, -1.0)
+                _assert_same(got, ref)
+    finally:
+        ctx.set_tuning(H.TUNE_DESCENT, 0)
+        tree.use_dense = True
+
+
 def test_descend_wide_candidate_lists_match_oracle():
     """Candidate lists of 10 (wider than the 5 of a five-wave pass): the
     8-wide passes, two per list, give the oracle's paths and scores."""
