@@ -41,11 +41,15 @@ constexpr int kDescGSmall = 5, kDescG = 8;
 constexpr int kDescThreads = 256;
 constexpr int kMaxBeam = 16;
 constexpr int kMaxLevels = 32;
-// bucket counters are kept in kDescCopies copies per key (workgroup b uses copy
-// b % kDescCopies) so that few hot keys -- 40 phyla for 3M pairs at level 1 --
+// bucket counters are kept in `copies` copies per key (workgroup b uses copy
+// b % copies) so that few hot keys -- 40 phyla for 3M pairs at level 1 --
 // do not serialise on one atomic; the scan runs over (key, copy) in key-major
-// order, so the pairs stay grouped by key
-constexpr int kDescCopies = 8;
+// order, so the pairs stay grouped by key.  64 copies while the counters stay
+// within 2^24 (a 76,040-category taxonomy, 3M pairs over 40 phyla: the level-1
+// count and scatter take 0.23 ms each, 1.3 ms with 8 copies), halved down to 8
+// for larger trees.
+constexpr int kDescCopiesMax = 64, kDescCopiesMin = 8;
+constexpr int64_t kDescMaxCounters = int64_t(1) << 24;
 
 typedef short short2_t __attribute__((ext_vector_type(2)));
 
@@ -119,13 +123,13 @@ __global__ void desc_init(Beams b, int n, int B) {
 // desc_score8; the others follow, scored by desc_score from the int16 rows.
 // Without the int8 rows (qmode null) every pair takes path 1.
 __device__ __forceinline__ int pair_bucket(int key, int r, int n_cat, const uint8_t* qmode,
-                                           const uint8_t* cflag, int copy) {
+                                           const uint8_t* cflag, int copies) {
     const int path = (qmode && qmode[r] == 0 && cflag[key]) ? 0 : 1;
-    return (path * (n_cat + 1) + key) * kDescCopies + copy;
+    return (path * (n_cat + 1) + key) * copies + (int)(blockIdx.x % (unsigned)copies);
 }
 
 __global__ void desc_hist(Beams b, int n, int B, int n_cat, const uint8_t* __restrict__ qmode,
-                          const uint8_t* __restrict__ cflag, int32_t* hist) {
+                          const uint8_t* __restrict__ cflag, int copies, int32_t* hist) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     const int nb = r < n ? b.nb[r] : 0;
     for (int j = 0; j < B; ++j) {
@@ -133,8 +137,7 @@ __global__ void desc_hist(Beams b, int n, int B, int n_cat, const uint8_t* __res
         int bucket = 0;
         if (act) {
             const int c = b.cat[(int64_t)r * B + j];
-            bucket = pair_bucket(c < 0 ? n_cat : c, r, n_cat, qmode, cflag,
-                                 (int)(blockIdx.x % kDescCopies));
+            bucket = pair_bucket(c < 0 ? n_cat : c, r, n_cat, qmode, cflag, copies);
         }
         (void)wave_agg_add(hist, bucket, act);
     }
@@ -220,8 +223,8 @@ desc_scan_apply(const int32_t* __restrict__ hist, int n_keys, const int32_t* __r
 }
 
 __global__ void desc_scatter(Beams b, int n, int B, int n_cat, const uint8_t* __restrict__ qmode,
-                             const uint8_t* __restrict__ cflag, int32_t* cursor, int32_t* pairs,
-                             int32_t* pkey) {
+                             const uint8_t* __restrict__ cflag, int copies, int32_t* cursor,
+                             int32_t* pairs, int32_t* pkey) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     const int nb = r < n ? b.nb[r] : 0;
     for (int j = 0; j < B; ++j) {
@@ -230,7 +233,7 @@ __global__ void desc_scatter(Beams b, int n, int B, int n_cat, const uint8_t* __
         if (act) {
             const int c = b.cat[(int64_t)r * B + j];
             key = c < 0 ? n_cat : c;
-            bucket = pair_bucket(key, r, n_cat, qmode, cflag, (int)(blockIdx.x % kDescCopies));
+            bucket = pair_bucket(key, r, n_cat, qmode, cflag, copies);
         }
         const int pos = wave_agg_add(cursor, bucket, act);
         if (act) {
@@ -1024,7 +1027,10 @@ int desc_launch(hvk_ctx* ctx, const DescPlan& pl, const int16_t* d_q_counts,
     const size_t beam_bytes = align_up(n * 4) + align_up(n * B * 4) + 2 * align_up(n * B * L * 4);
     const size_t sc_bytes = align_up((size_t)n * B * Fmax * 4);
     const size_t pair_bytes = 2 * align_up((size_t)n * B * 4);
-    const int n_keys = 2 * (n_cat + 1) * kDescCopies;  // counter slots: (path, key, copy)
+    int copies = kDescCopiesMax;
+    while (copies > kDescCopiesMin && int64_t(2) * (n_cat + 1) * copies > kDescMaxCounters)
+        copies /= 2;
+    const int n_keys = 2 * (n_cat + 1) * copies;  // counter slots: (path, key, copy)
     const int n_scan = (n_keys + kScanTile - 1) / kScanTile;  // scan tiles
     const size_t key_bytes = 2 * align_up((size_t)n_keys * 4) + 256 + align_up((size_t)n_scan * 4);
     const size_t sort_bytes = 2 * align_up((size_t)n * maxN * 4);
@@ -1095,7 +1101,7 @@ int desc_launch(hvk_ctx* ctx, const DescPlan& pl, const int16_t* d_q_counts,
     }
     // after the scatter, the cursor of the last path-0 bucket holds the
     // number of path-0 pairs: the first pair of desc_score
-    const int32_t* n_i8 = cursor + (n_cat + 1) * kDescCopies - 1;
+    const int32_t* n_i8 = cursor + (n_cat + 1) * copies - 1;
     HVK_HIP(hipGetLastError());
     int cur = 0;
     for (int lvl = 0; lvl < L; ++lvl) {
@@ -1109,14 +1115,14 @@ int desc_launch(hvk_ctx* ctx, const DescPlan& pl, const int16_t* d_q_counts,
             hipLaunchKernelGGL(desc_fill, dim3((n_keys + kDescThreads - 1) / kDescThreads),
                                dim3(kDescThreads), 0, st, hist, (int64_t)n_keys, 0);
             hipLaunchKernelGGL(desc_hist, dim3(rb), dim3(kDescThreads), 0, st, bm[cur], n_q, B,
-                               n_cat, i8.qmode, i8.cflag, hist);
+                               n_cat, i8.qmode, i8.cflag, copies, hist);
             hipLaunchKernelGGL(desc_scan_sums, dim3(n_scan), dim3(1024), 0, st, hist, n_keys,
                                scan_part);
             hipLaunchKernelGGL(desc_scan_top, dim3(1), dim3(1024), 0, st, scan_part, n_scan, total);
             hipLaunchKernelGGL(desc_scan_apply, dim3(n_scan), dim3(1024), 0, st, hist, n_keys,
                                scan_part, cursor);
             hipLaunchKernelGGL(desc_scatter, dim3(rb), dim3(kDescThreads), 0, st, bm[cur], n_q, B,
-                               n_cat, i8.qmode, i8.cflag, cursor, pairs, pkey);
+                               n_cat, i8.qmode, i8.cflag, copies, cursor, pairs, pkey);
             // the longest candidate list of this level: the roots at level 0
             const bool narrow = (lvl == 0 ? pl.n_roots : max_span) <= kDescGSmall;
             if (use_i8) {

@@ -713,7 +713,7 @@ def run_descent(a):
                   "top-3 beam per rank)",
         "value": nq / sec, "unit": "reads/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": 1000.0 * sec, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "int16",
+        "vs_baseline": None, "dtype": "int8/int16",
         "data": "synthetic (deterministic generator, seed 0)",
         "config": {"workload": f"config 4: 7-rank taxonomy 1-40-200-800-3000-12000-60000, "
                                f"leaves 1500 nt, {nq} reads 150-1600 nt, beam {beam}",
