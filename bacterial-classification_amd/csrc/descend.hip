@@ -614,9 +614,13 @@ desc_score8(TreeDev t, const int8_t* __restrict__ q8, const int64_t* __restrict_
 // the fewest for the roots' largest |p|).  v_mfma_i32_32x32x32_i8 multiplies
 // digit pairs; the 2 ND products fall into ND + 1 weights 2^(8w) (w = a + b),
 // each accumulated in int32 -- at most 2 products x 10240 x 128 x 128 < 2^31
-// -- and combined in int64 at the end: the exact dot.  A wave owns 32 reads; the workgroup (4 waves) stages each 32-element
-// K step of the roots' digits in LDS once.  Lane l holds row / column l % 32
-// and K bytes [16 (l / 32), +16) of both operands, as the MFMA pairs them.
+// -- and combined in int64 at the end: the exact dot.  A wave owns 32 reads;
+// the workgroup (4 waves) stages each 32-element K step of the roots' digits
+// (split once per launch by desc_root_digits) in LDS, double-buffered with one
+// barrier a step, and every step's loads go out a step ahead.  Lane l holds
+// row / column l % 32 and K bytes [16 (l / 32), +16) of both operands, as the
+// MFMA pairs them.  (Staging the read rows through LDS too, so that each load
+// reads whole lines: 8.4 -> 11.4 ms per 1M reads, not kept.)
 typedef int v4i_t __attribute__((ext_vector_type(4)));
 typedef int v16i_t __attribute__((ext_vector_type(16)));
 constexpr int kDenseN = 64;     // roots (two 32-wide blocks)
@@ -628,15 +632,56 @@ __device__ __forceinline__ uint32_t pack4(int a, int b, int c, int d) {
            ((uint32_t)(d & 0xFF) << 24);
 }
 
+// The roots' prototype rows as ND signed base-256 digits for desc_dense, in
+// the order its workgroups stage them: for each 32-element K step s, digit g
+// and root slot c, the 32 bytes of digit g of p[roots[c]][32 s, 32 s + 32)
+// (p = d0 + 256 d1 + ...), zero for c >= n_roots up to kDenseN.  A step's
+// staging then reads ND contiguous 2 KB blocks.  One thread per 8 elements.
+template <int ND>
+__global__ void desc_root_digits(TreeDev t, int64_t K, int8_t* __restrict__ droot) {
+    const int64_t j = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 8;
+    if (j >= (int64_t)kDenseN * K) return;
+    const int c = (int)(j / K);
+    const int64_t k = j - (int64_t)c * K;
+    const int cat = c < t.n_roots ? t.roots[c] : -1;
+    int4 lv = make_int4(0, 0, 0, 0), hv = make_int4(0, 0, 0, 0);
+    if (cat >= 0) {
+        lv = *reinterpret_cast<const int4*>(t.lo + (int64_t)cat * t.ld + k);
+        if (ND > 2) hv = *reinterpret_cast<const int4*>(t.hi + (int64_t)cat * t.ld + k);
+    }  // ND = 2: every |p| <= 32639, so the hi limb is zero
+    const int lw[4] = {lv.x, lv.y, lv.z, lv.w}, hw[4] = {hv.x, hv.y, hv.z, hv.w};
+    int d[ND][8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int lo = (int)(int16_t)(lw[e >> 1] >> (16 * (e & 1)));
+        const int hi = (int)(int16_t)(hw[e >> 1] >> (16 * (e & 1)));
+        int p = lo + hi * 65536;
+#pragma unroll
+        for (int g = 0; g + 1 < ND; ++g) {
+            d[g][e] = (int)(int8_t)p;
+            p = (p - d[g][e]) >> 8;
+        }
+        d[ND - 1][e] = p;
+    }
+    const int64_t step = k >> 5;
+#pragma unroll
+    for (int g = 0; g < ND; ++g)
+        *reinterpret_cast<uint2*>(droot + ((step * ND + g) * kDenseN + c) * 32 + (k & 31)) =
+            make_uint2(pack4(d[g][0], d[g][1], d[g][2], d[g][3]),
+                       pack4(d[g][4], d[g][5], d[g][6], d[g][7]));
+}
+
 // (Two workgroups per CU: the accumulator registers -- 160 at ND = 4 -- and
 // the rest fit in 256, and the second wave per SIMD hides the read-row loads.)
 template <int ND>
 __global__ void __launch_bounds__(256, 2)
 desc_dense(TreeDev t, const int16_t* __restrict__ q, int64_t qld, const int64_t* __restrict__ qn,
-           int n, int B, int64_t K, int Fmax, float* __restrict__ sc, int8_t* __restrict__ q8,
-           uint8_t* __restrict__ qmode) {
+           int n, int B, int64_t K, int Fmax, float* __restrict__ sc,
+           const int8_t* __restrict__ droot, int8_t* __restrict__ q8, uint8_t* __restrict__ qmode) {
     static_assert(ND >= 2 && ND <= 4, "prototype digits");
-    __shared__ __attribute__((aligned(16))) int8_t bl[ND][kDenseN][kDenseRow];
+    // the roots' digits of one 32-element K step, double-buffered: step s reads
+    // buffer s & 1 while it stages step s + 1 in the other, one barrier a step
+    __shared__ __attribute__((aligned(16))) int8_t bl[2][ND][kDenseN][kDenseRow];
     // the int8 rows pass through LDS, 128 K bytes of the wave's 32 reads at a
     // time, so that each store writes whole 128-byte lines (8 rows x 128 B)
     __shared__ __attribute__((aligned(16))) int8_t qst[4][32][kDenseStage];
@@ -644,53 +689,44 @@ desc_dense(TreeDev t, const int16_t* __restrict__ q, int64_t qld, const int64_t*
     const int r0 = (blockIdx.x * 4 + wave) * 32;
     const int i = lane & 31, kg = lane >> 5;
     const int16_t* __restrict__ qrow = q + (int64_t)min(r0 + i, n - 1) * qld;  // rows past n: clamped
-    // staging: thread -> (root bc, 8 elements at bk) of the K step
+    // staging: thread -> (root bc, 8 elements at bk) of the K step, from the
+    // roots' digits in step order (desc_root_digits; slots past the roots are
+    // zero): 8 bytes at tid * 8 of each digit's 2 KB block
     const int bc = tid >> 2, bk = (tid & 3) * 8;
-    const int catB = bc < t.n_roots ? t.roots[bc] : -1;
-    const int16_t* __restrict__ blo = t.lo + (int64_t)(catB < 0 ? 0 : catB) * t.ld;
-    const int16_t* __restrict__ bhi = t.hi + (int64_t)(catB < 0 ? 0 : catB) * t.ld;
+    auto load_b = [&](int64_t k0, uint2 (&dv)[ND]) {
+        const int8_t* blk = droot + (k0 >> 5) * ND * (kDenseN * 32) + tid * 8;
+#pragma unroll
+        for (int g = 0; g < ND; ++g)
+            dv[g] = *reinterpret_cast<const uint2*>(blk + g * (kDenseN * 32));
+    };
+    auto stage_b = [&](int buf, const uint2 (&dv)[ND]) {
+#pragma unroll
+        for (int g = 0; g < ND; ++g) *reinterpret_cast<uint2*>(&bl[buf][g][bc][bk]) = dv[g];
+    };
     v16i_t acc[ND + 1][2];
 #pragma unroll
     for (int w = 0; w < ND + 1; ++w)
 #pragma unroll
         for (int nb = 0; nb < 2; ++nb) acc[w][nb] = v16i_t{};
     int qbad = 0;  // nonzero when some q of this lane's part of the row is outside int8
+    // the read's K bytes [16 kg, +16) of the current step, and the roots' slabs
+    int4 qa = *reinterpret_cast<const int4*>(qrow + 16 * kg);
+    int4 qb = *reinterpret_cast<const int4*>(qrow + 16 * kg + 8);
+    {
+        uint2 dv[ND];
+        load_b(0, dv);
+        stage_b(0, dv);
+    }
+    __syncthreads();
     for (int64_t k0 = 0; k0 < K; k0 += 32) {
-        __syncthreads();  // the previous step's digits are consumed
-        {
-            int d[ND][8];
-            if (catB >= 0) {
-                const int4 lv = *reinterpret_cast<const int4*>(blo + k0 + bk);
-                const int4 hv = ND > 2 ? *reinterpret_cast<const int4*>(bhi + k0 + bk)
-                                       : make_int4(0, 0, 0, 0);  // |p| < 2^15: hi is zero
-                const int lw[4] = {lv.x, lv.y, lv.z, lv.w}, hw[4] = {hv.x, hv.y, hv.z, hv.w};
-#pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const int lo = (int)(int16_t)(lw[e >> 1] >> (16 * (e & 1)));
-                    const int hi = (int)(int16_t)(hw[e >> 1] >> (16 * (e & 1)));
-                    int p = lo + hi * 65536;
-#pragma unroll
-                    for (int g = 0; g + 1 < ND; ++g) {
-                        d[g][e] = (int)(int8_t)p;
-                        p = (p - d[g][e]) >> 8;
-                    }
-                    d[ND - 1][e] = p;
-                }
-            } else {
-#pragma unroll
-                for (int g = 0; g < ND; ++g)
-#pragma unroll
-                    for (int e = 0; e < 8; ++e) d[g][e] = 0;
-            }
-#pragma unroll
-            for (int g = 0; g < ND; ++g)
-                *reinterpret_cast<uint2*>(&bl[g][bc][bk]) =
-                    make_uint2(pack4(d[g][0], d[g][1], d[g][2], d[g][3]),
-                               pack4(d[g][4], d[g][5], d[g][6], d[g][7]));
-        }
-        // the read's digits for K bytes [16 kg, +16) of this step
-        const int4 qa = *reinterpret_cast<const int4*>(qrow + k0 + 16 * kg);
-        const int4 qb = *reinterpret_cast<const int4*>(qrow + k0 + 16 * kg + 8);
+        const int buf = (int)(k0 >> 5) & 1;
+        // the next step's loads go out before this step's work (the last step
+        // reloads its own slabs: in bounds, unused)
+        const int64_t kn = k0 + 32 < K ? k0 + 32 : k0;
+        const int4 qan = *reinterpret_cast<const int4*>(qrow + kn + 16 * kg);
+        const int4 qbn = *reinterpret_cast<const int4*>(qrow + kn + 16 * kg + 8);
+        uint2 dvn[ND];
+        load_b(kn, dvn);
         const int qw[8] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w};
         int q0[16], q1[16];
 #pragma unroll
@@ -719,17 +755,21 @@ desc_dense(TreeDev t, const int16_t* __restrict__ q, int64_t qld, const int64_t*
                 }
             }
         }
-        __syncthreads();  // the step's digits are staged
 #pragma unroll
         for (int nb = 0; nb < 2; ++nb)
 #pragma unroll
             for (int g = 0; g < ND; ++g) {
-                const v4i_t b = *reinterpret_cast<const v4i_t*>(&bl[g][nb * 32 + i][16 * kg]);
+                const v4i_t b = *reinterpret_cast<const v4i_t*>(&bl[buf][g][nb * 32 + i][16 * kg]);
 #pragma unroll
                 for (int qa_i = 0; qa_i < 2; ++qa_i)
                     acc[qa_i + g][nb] =
                         __builtin_amdgcn_mfma_i32_32x32x32_i8(a[qa_i], b, acc[qa_i + g][nb], 0, 0, 0);
             }
+        // every wave is past step s - 1, so the other buffer is free
+        stage_b(buf ^ 1, dvn);
+        __syncthreads();  // step s + 1 staged; this step's buffer consumed
+        qa = qan;
+        qb = qbn;
     }
     if (q8) {
         qbad |= __shfl_xor(qbad, 32);  // lanes i and i + 32 hold one read
@@ -1042,8 +1082,10 @@ int desc_launch(hvk_ctx* ctx, const DescPlan& pl, const int16_t* d_q_counts,
                                          2 * align_up((size_t)n_cat * K) + align_up((size_t)n_cat) +
                                          align_up((size_t)n_cat + 1)
                                    : 0;
+    // the roots' digit planes of the dense level 0 (at most 4 digits)
+    const size_t root_bytes = dense ? align_up((size_t)4 * kDenseN * K) : 0;
     if (int e = ctx->ws_desc.ensure(2 * beam_bytes + sc_bytes + pair_bytes + key_bytes + sort_bytes +
-                                    i8_bytes))
+                                    i8_bytes + root_bytes))
         return e;
     char* w = (char*)ctx->ws_desc.ptr;
     auto take = [&](size_t bytes) { char* p = w; w += align_up(bytes); return p; };
@@ -1063,6 +1105,7 @@ int desc_launch(hvk_ctx* ctx, const DescPlan& pl, const int16_t* d_q_counts,
     int32_t* scan_part = (int32_t*)take((size_t)n_scan * 4);
     float* ws_s = (float*)take((size_t)n * maxN * 4);
     int32_t* ws_c = (int32_t*)take((size_t)n * maxN * 4);
+    int8_t* droot = dense ? (int8_t*)take((size_t)4 * kDenseN * K) : nullptr;
     DescI8 i8{};
     if (use_i8) {
         i8.q8 = (int8_t*)take((size_t)n * K);
@@ -1106,11 +1149,16 @@ int desc_launch(hvk_ctx* ctx, const DescPlan& pl, const int16_t* d_q_counts,
     int cur = 0;
     for (int lvl = 0; lvl < L; ++lvl) {
         if (lvl == 0 && dense) {  // every read has one entry, the root
-            auto dense_k = pl.max_abs_proto <= 32639     ? desc_dense<2>
-                           : pl.max_abs_proto <= 8355711 ? desc_dense<3>
-                                                         : desc_dense<4>;
+            const int nd = pl.max_abs_proto <= 32639 ? 2 : pl.max_abs_proto <= 8355711 ? 3 : 4;
+            auto digits_k = nd == 2 ? desc_root_digits<2>
+                            : nd == 3 ? desc_root_digits<3>
+                                      : desc_root_digits<4>;
+            auto dense_k = nd == 2 ? desc_dense<2> : nd == 3 ? desc_dense<3> : desc_dense<4>;
+            const int64_t n8 = (int64_t)kDenseN * K / 8;
+            hipLaunchKernelGGL(digits_k, dim3((unsigned)((n8 + 255) / 256)), dim3(256), 0, st, t, K,
+                               droot);
             hipLaunchKernelGGL(dense_k, dim3((n_q + 127) / 128), dim3(256), 0, st, t, d_q_counts,
-                               q_ld, d_q_sqnorm, n_q, B, K, Fmax, sc, i8.q8, i8.qmode);
+                               q_ld, d_q_sqnorm, n_q, B, K, Fmax, sc, droot, i8.q8, i8.qmode);
         } else {
             hipLaunchKernelGGL(desc_fill, dim3((n_keys + kDescThreads - 1) / kDescThreads),
                                dim3(kDescThreads), 0, st, hist, (int64_t)n_keys, 0);
