@@ -179,6 +179,53 @@ def test_descend_int8_and_int16_rows_match_oracle(synth):
         tree.use_dense = True
 
 
+def test_descend_int8_row_boundary_counts(synth):
+    """Count rows given directly, with entries at the edges of int8 (-128 and
+    127 fit, -129 and 128 do not) and far beyond it: a read takes the int8 rows
+    exactly when every count fits, and every read's descent equals the
+    oracle's, for both level-0 forms and both settings."""
+    import hdc_amd as H
+    import torch
+    ctx, tax, tree, q, proto, qc, truth = synth
+    D = proto.shape[1]
+    rng = np.random.default_rng(5)
+    base = qc[:8].astype(np.int32)
+    groups = []
+    for v in (-128, 127, 128, -129, 300, -20000):
+        x = np.clip(base, -100, 100)
+        x[:, rng.integers(0, D, 3)] = v
+        groups.append(x)
+    groups.append(np.clip(base, -127, 127))
+    counts = np.concatenate(groups).astype(np.int16)
+    n = len(counts)
+
+    class Rows:
+        pass
+
+    rows = Rows()
+    rows.n = n
+    rows.counts = torch.zeros((n, ctx.ld), dtype=torch.int16, device="cuda")
+    rows.counts[:, :D] = torch.from_numpy(counts).cuda()
+    rows.sqnorm = torch.from_numpy((counts.astype(np.int64) ** 2).sum(axis=1)).cuda()
+    rows.maxabs = torch.from_numpy(np.abs(counts.astype(np.int32)).max(axis=1)).cuda()
+    fits = (counts.astype(np.int32) >= -128).all(axis=1) & (counts.astype(np.int32) <= 127).all(axis=1)
+    assert fits[:16].all() and not fits[16:48].any() and fits[48:].all()
+    ref = orc.descend(proto, tax.child_off, tax.child_idx, tax.roots, counts, 3, -1.0, 6)
+    try:
+        for dense in (True, False):
+            tree.use_dense = dense
+            for mode in (0, 1):
+                ctx.set_tuning(H.TUNE_DESCENT, mode)
+                print(f"descend boundary counts mode={mode} dense_level0={dense}", flush=True)
+                res = H.descend(ctx, tree, rows, beam=3, threshold=-1.0, levels=6)
+                torch.cuda.synchronize()
+                _assert_same((res.path.cpu().numpy(), res.score.cpu().numpy(),
+                              res.length.cpu().numpy(), res.exit_level.cpu().numpy()), ref)
+    finally:
+        ctx.set_tuning(H.TUNE_DESCENT, 0)
+        tree.use_dense = True
+
+
 def test_descend_wide_candidate_lists_match_oracle():
     """Candidate lists of 10 (wider than the 5 of a five-wave pass): the
     8-wide passes, two per list, give the oracle's paths and scores."""
