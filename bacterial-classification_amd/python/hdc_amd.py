@@ -33,7 +33,7 @@ EXPORTS = [
     "hvk_batch_create", "hvk_batch_destroy", "hvk_batch_config", "hvk_batch_get_config",
     "hvk_batch_classify", "hvk_batch_autotune", "hvk_batch_descend",
     "hvk_db_write", "hvk_db_open", "hvk_db_close", "hvk_db_info", "hvk_db_array", "hvk_db_upload",
-    "hvk_search_mfma_bounded_noalloc",
+    "hvk_search_mfma_bounded_noalloc", "hvk_calib_mfma_i8",
 ]
 
 DB_COUNTS, DB_LO, DB_HI, DB_CLAMP8, DB_SUMS = 1, 2, 4, 8, 16
@@ -161,6 +161,7 @@ def lib():
                                      ctypes.POINTER(ctypes.c_int), vp]
     L.hvk_batch_descend.argtypes = [vp, ctypes.POINTER(Tree), vp, vp, vp, i32, i32,
                                     ctypes.c_float, i32, vp, vp, vp, vp, vp]
+    L.hvk_calib_mfma_i8.argtypes = [i32, i32, i32, i32, vp, ctypes.POINTER(ctypes.c_double), vp]
     _lib = L
     return L
 

@@ -193,6 +193,42 @@ def library_int8_gemm(m=16384, n=16384, k=DIM, seconds=3.0):
             "exact": exact, "power": pw.summary()}
 
 
+def mfma_power_ceiling(q_sd, r_sd, seconds=4.0):
+    """The int8 MFMA rate this card sustains under its power and clock limits
+    on operands spread like the search's (hvk_calib_mfma_i8: MFMAs from
+    registers, operands ~ round(N(0, sd)) clamped to +-127, no memory
+    traffic, two waves per SIMD), timed with CUDA events on torch's current
+    stream, which the launch uses; power and clock on an untimed repeat."""
+    import torch
+    import hdc_amd as H
+    L = H.lib()
+    sink = torch.zeros(1, dtype=torch.int32, device="cuda")
+    ops = H.ctypes.c_double()
+
+    def launch(iters):
+        H.check(L.hvk_calib_mfma_i8(2, iters, int(q_sd), int(r_sd), H._ptr(sink),
+                                    H.ctypes.byref(ops), H._stream()), "hvk_calib_mfma_i8")
+
+    launch(64)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    launch(2048)
+    torch.cuda.synchronize()
+    iters = int(min(1 << 24, max(64, 2048 * seconds / max(time.time() - t0, 1e-4))))
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    launch(iters)
+    e.record()
+    torch.cuda.synchronize()
+    ms = s.elapsed_time(e)
+    with PowerSampler() as pw:
+        launch(iters)
+        torch.cuda.synchronize()
+    return {"op": "hvk_calib_mfma_i8: v_mfma_i32_32x32x32_i8 from registers, 2 waves per SIMD",
+            "q_sd": int(q_sd), "r_sd": int(r_sd), "ms": ms,
+            "tops": ops.value / (ms / 1000.0) / 1e12, "power": pw.summary()}
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -209,6 +245,8 @@ def parse():
     ap.add_argument("--no-two-limb", action="store_true")
     ap.add_argument("--no-gemm-ref", action="store_true",
                     help="skip the library int8 GEMM reference point")
+    ap.add_argument("--no-ceiling", action="store_true",
+                    help="skip the power-capped int8 MFMA calibration (hvk_calib_mfma_i8)")
     ap.add_argument("--no-power", action="store_true",
                     help="no amd-smi power sampling (for runs under rocprofv3)")
     ap.add_argument("--workload", choices=["classify", "precision", "descent", "batch", "config0"],
@@ -377,6 +415,18 @@ def main():
         print(f"[bench] library int8 GEMM: {gemm['tops']:.0f} TOPS (search kernel "
               f"{achieved_tops:.0f} TOPS)", file=sys.stderr, flush=True)
 
+    # the card's int8 MFMA rate under its power cap on operands spread like
+    # these (standard deviation of the clamped rows, over a sample of rows)
+    ceiling = None
+    if not a.no_ceiling:
+        q_sd = float(reads.clamp8[: min(nq, 4096)].float().std().item())
+        r_sd = float(db.clamp8[: min(n_refs, 4096)].float().std().item())
+        ceiling = mfma_power_ceiling(round(q_sd), round(r_sd))
+        ceiling["search_tops_over_ceiling"] = achieved_tops / ceiling["tops"]
+        print(f"[bench] MFMA ceiling on sd ({round(q_sd)}, {round(r_sd)}) operands: "
+              f"{ceiling['tops']:.0f} TOPS (search kernel {achieved_tops:.0f} TOPS)",
+              file=sys.stderr, flush=True)
+
     # plain form on a subset: median of plain_runs timed runs
     plain = None
     if not a.no_plain:
@@ -433,7 +483,8 @@ def main():
                      "kernel": bounded_kernel,
                      "kernel_ms": float(np.mean(srch_ms)),
                      "power": power.summary(),
-                     "library_int8_gemm": gemm},
+                     "library_int8_gemm": gemm,
+                     "power_capped_ceiling": ceiling},
         "encode": encode_line(enc_ms, enc_gbs, nq, lens),
         "mfma_two_limb": two,
         "plain": plain,

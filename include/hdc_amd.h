@@ -70,6 +70,18 @@ int hvk_base_hv(const hvk_ctx* ctx, char symbol, uint64_t* host_out);
 int hvk_ctx_set_timing(hvk_ctx* ctx, int enable);
 int hvk_ctx_last_timing(hvk_ctx* ctx, float* encode_ms, float* search_ms);
 
+/* Calibration (no counterpart in the reference; for measurement only): runs
+ * int8 MFMAs (v_mfma_i32_32x32x32_i8) from registers on every CU of the
+ * current device, waves_per_simd (1 or 2) waves per SIMD, iters rounds of 64
+ * MFMAs per wave, on operands drawn ~ round(N(0, q_sd)) and round(N(0, r_sd))
+ * clamped to +-127 (sd 0: all-zero operands), with no memory traffic.
+ * *ops (nullable) receives the integer operations launched (2 per MAC).
+ * d_sink: one device int32 that keeps the products live.  Time the launch
+ * with events on `stream` to get the card's int8 MFMA rate for that operand
+ * distribution under its power and clock limits. */
+int hvk_calib_mfma_i8(int waves_per_simd, int iters, int q_sd, int r_sd, int32_t* d_sink,
+                      double* ops, hvk_stream stream);
+
 /* Tuning of the encode and search launches (no counterpart in the
  * reference; used by the tile autotuner and the tests).  Results do not
  * depend on it.

@@ -133,3 +133,13 @@ def test_cli_usage_errors_throw():
     assert p.returncode != 0 and "Usage: identify" in p.stderr
     p = subprocess.run([os.path.join(b, "propogate-children")], capture_output=True, text=True)
     assert p.returncode != 0 and "Usage: propogate-children" in p.stderr
+
+
+def test_calib_rejects_bad_args_without_device():
+    L = H.lib()
+    sink = ctypes.c_void_p(16)  # never dereferenced: every call below fails validation first
+    ops = ctypes.c_double()
+    for wps, iters, qs, rs, snk in ((0, 1, 15, 39, sink), (3, 1, 15, 39, sink), (1, 0, 15, 39, sink),
+                                    (1, 1, -1, 39, sink), (1, 1, 15, 128, sink), (2, 8, 15, 39, None),
+                                    (1, (1 << 24) + 1, 15, 39, sink)):
+        assert L.hvk_calib_mfma_i8(wps, iters, qs, rs, snk, ctypes.byref(ops), None) == H.HVK_ERR_ARG
