@@ -169,6 +169,10 @@ int hvk_ctx_set_tuning(hvk_ctx* ctx, int param, int value) {
             if (value < 0 || value > 1) return HVK_ERR_ARG;
             ctx->encode_legacy = value == 1;
             return HVK_OK;
+        case HVK_TUNE_GROUP_SLICES:
+            if (value < 0 || value > 64) return HVK_ERR_ARG;
+            ctx->tune_group_slices = value;
+            return HVK_OK;
         case HVK_TUNE_DESCENT:
             if (value < 0 || value > 1) return HVK_ERR_ARG;
             ctx->tune_desc_i16 = value == 1;

@@ -74,6 +74,7 @@ struct hvk_ctx {
     int tune_shape = 0;       // tile shape of the bounded search (8-entry lists)
     bool encode_legacy = false;  // workgroup-per-sequence encoders only
     bool tune_desc_i16 = false;  // descent: int16 read rows only
+    int tune_group_slices = 0;   // reference slices per XCD group of the MFMA search, 0 = auto
 };
 
 // Kernel launchers (defined in the .hip files).

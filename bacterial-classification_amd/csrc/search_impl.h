@@ -363,7 +363,9 @@ struct TileMap {
     int n_splits;     // reference slices
     int split_tiles;  // reference tiles of NT per slice
     int gq;           // query tiles per XCD group
-    int rx;           // workgroups per XCD group = gq * n_splits
+    int sg;           // reference slices per XCD group (sg <= n_splits)
+    int n_sgroups;    // slice groups = ceil(n_splits / sg)
+    int rx;           // workgroups per XCD group = gq * sg
 };
 
 // 16-byte global -> LDS copy (global_load_lds_dwordx4): lane l of the wave
@@ -476,10 +478,11 @@ mfma_kernel(const int8_t* __restrict__ qlo, const int8_t* __restrict__ qhi,
     // XCD-grouped tile of this workgroup (all branches below are uniform)
     const int b = blockIdx.x;
     const int slot = b / kXcds, gi = slot / map.rx, w = slot - gi * map.rx;
-    const int g = gi * kXcds + b % kXcds;
-    const int qt = g * map.gq + w / map.n_splits;
-    const int split = w % map.n_splits;
-    if (qt >= map.q_tiles) return;
+    const int g = gi * kXcds + b % kXcds;  // group: gq query tiles x sg slices
+    const int qg = g / map.n_sgroups, sgi = g - qg * map.n_sgroups;
+    const int qt = qg * map.gq + w / map.sg;
+    const int split = sgi * map.sg + w % map.sg;
+    if (qt >= map.q_tiles || split >= map.n_splits) return;
     const int m0 = qt * MT;
     if constexpr (IND) {
         const int c = *q_count;
@@ -1239,12 +1242,12 @@ inline int choose_splits(hvk_ctx* ctx, int q_tiles, int r_tiles, int wg_per_cu) 
 }
 
 // Reference slices and XCD groups of an MFMA launch (one workgroup per CU).
-// Automatic choice: 4 slices, or as many as fill the device when there are
-// few query tiles (measured at 64k-1M reads x 100k references, D = 10240:
-// 4 slices beat 1, 2, 8 and 16 by 2-15%; every slice rebuilds its candidate
-// lists from an empty start, while one slice leaves the last round of
-// workgroups part-empty).  want_splits > 0 (the exact second pass) or the
+// Automatic choice: 16 slices from 2048 query tiles up, else 8, or as many as
+// fill the device when there are few query tiles (every slice rebuilds its
+// candidate lists from an empty start, while few slices leave the last round
+// of workgroups part-empty).  want_splits > 0 (the exact second pass) or the
 // context's tuning value forces the slice count.
+constexpr int kGroupSlices = 8;  // reference slices per XCD group by default
 struct MfmaPlan {
     TileMap map;
     int blocks;
@@ -1266,9 +1269,18 @@ inline MfmaPlan plan_mfma(hvk_ctx* ctx, int q_tiles, int r_tiles, int want_split
     p.map.q_tiles = q_tiles;
     p.map.split_tiles = (r_tiles + s - 1) / s;
     p.map.n_splits = (r_tiles + p.map.split_tiles - 1) / p.map.split_tiles;
-    p.map.gq = per_xcd / p.map.n_splits > 0 ? per_xcd / p.map.n_splits : 1;
-    p.map.rx = p.map.gq * p.map.n_splits;
-    const int groups = (q_tiles + p.map.gq - 1) / p.map.gq;
+    // XCD groups of gq query tiles x sg slices (gq * sg = CUs per XCD): the
+    // XCD's L2 serves each query slab to sg workgroups and each reference
+    // slab to gq.  Measured at 1M reads x 100k references with 16 slices:
+    // 8 slices per group (4 x 8) 905 ms, 4 (8 x 4) 907 ms, all 16 (2 x 16)
+    // 920 ms, 2 (16 x 2) 928-968 ms; identical results.
+    int sg = ctx->tune_group_slices > 0 ? ctx->tune_group_slices : kGroupSlices;
+    if (sg > p.map.n_splits) sg = p.map.n_splits;
+    p.map.sg = sg;
+    p.map.n_sgroups = (p.map.n_splits + sg - 1) / sg;
+    p.map.gq = per_xcd / sg > 0 ? per_xcd / sg : 1;
+    p.map.rx = p.map.gq * sg;
+    const int groups = (q_tiles + p.map.gq - 1) / p.map.gq * p.map.n_sgroups;
     p.blocks = (groups + kXcds - 1) / kXcds * kXcds * p.map.rx;
     return p;
 }

@@ -46,6 +46,7 @@ TUNE_SELECT_KIB = 2
 TUNE_SHAPE = 3
 TUNE_ENCODER = 4
 TUNE_DESCENT = 5
+TUNE_GROUP_SLICES = 6
 
 
 class HvkError(RuntimeError):

@@ -99,12 +99,18 @@ int hvk_calib_mfma_i8(int waves_per_simd, int iters, int q_sd, int r_sd, int32_t
  * every count fits int8 from an int8 copy of its row, against candidates
  * whose every |p| <= 32639 split into two int8 digits (the default); other
  * pairs use the int16 rows.  1 = int16 rows only.  The scores are the same
- * exact dots either way. */
+ * exact dots either way.
+ * HVK_TUNE_GROUP_SLICES: reference slices that share one XCD's group of
+ * workgroups in the MFMA search (the group holds CUs-per-XCD / slices query
+ * tiles, so that the XCD's L2 serves each query tile to that many slices and
+ * each reference tile to that many query tiles); 0 = automatic (8, or all
+ * slices when fewer), else 1..64. */
 #define HVK_TUNE_SPLITS 1
 #define HVK_TUNE_SELECT_KIB 2
 #define HVK_TUNE_SHAPE 3
 #define HVK_TUNE_ENCODER 4
 #define HVK_TUNE_DESCENT 5
+#define HVK_TUNE_GROUP_SLICES 6
 int hvk_ctx_set_tuning(hvk_ctx* ctx, int param, int value);
 
 /* ---- packing (host) -----------------------------------------------------

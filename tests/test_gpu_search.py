@@ -305,3 +305,45 @@ def test_bounded_tile_shapes(workload, shape):
                                      encoded_from_counts(big, ctx.ld), 5), ref_idx, ref_cos, 5)
     finally:
         ctx.set_tuning(H.TUNE_SHAPE, 0)
+
+
+@pytest.fixture(scope="module")
+def wide_workload():
+    """2,000 references (8 reference tiles of 256) and 300 reads, so that 8
+    slices can be grouped per XCD in several ways."""
+    import hdc_amd as H
+    import torch
+    spec = H.make_spec(n_ancestors=40, desc_per_ancestor=50, anc_len_min=1300, anc_len_max=1600,
+                       div_min=0.02, div_max=0.08, n_reads=300, read_len_min=150,
+                       read_len_max=300, err_min=0.005, err_max=0.02)
+    w = H.generate(spec)
+    ctx = H.Context(DIM, K)
+    refs = H.encode(ctx, H.DeviceSeqs(w.refs, w.ref_offsets), clamp=True)
+    reads = H.encode(ctx, H.DeviceSeqs(w.reads, w.read_offsets), clamp=True)
+    torch.cuda.synchronize()
+    rc = refs.counts.cpu().numpy()[:, :DIM]
+    qc = reads.counts.cpu().numpy()[:, :DIM]
+    return ctx, refs, reads, rc, qc
+
+
+@pytest.mark.parametrize("form", ["bounded", "two_limb"])
+def test_xcd_slice_groups(wide_workload, form):
+    """8 reference slices grouped 1, 3 (a partial last group), 8 or
+    automatically per XCD (HVK_TUNE_GROUP_SLICES) give the oracle's top-5."""
+    import hdc_amd as H
+    ctx, refs, reads, rc, qc = wide_workload
+    assert refs.n == 2000
+    ref_idx, ref_cos = orc.topk16(qc, rc, 5)
+    ctx.set_tuning(H.TUNE_SPLITS, 8)
+    try:
+        for sg in (1, 3, 8, 0):
+            print(f"{form} slices=8 group_slices={sg}", flush=True)
+            ctx.set_tuning(H.TUNE_GROUP_SLICES, sg)
+            if form == "bounded":
+                res = H.search_mfma_bounded(ctx, reads, refs, 5)
+            else:
+                res = H.search_mfma(ctx, reads, refs, 5, q_two_limbs=True, r_two_limbs=True)
+            _check(res, ref_idx, ref_cos, 5)
+    finally:
+        ctx.set_tuning(H.TUNE_GROUP_SLICES, 0)
+        ctx.set_tuning(H.TUNE_SPLITS, 0)
