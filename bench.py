@@ -15,6 +15,10 @@ untimed steps (amd-smi), the vendor library's int8 GEMM (torch._int_mm on
 hipBLASLt) at K = D on the same card, and the card's int8 MFMA rate on
 operands spread like the search's rows with no memory traffic
 (hvk_calib_mfma_i8), the two rates this card sustains under its power cap.
+After the headline line is measured, the default run also records the
+batching form (1M reads of 100-1,600 nt) and the hierarchical descent
+(262,144 reads) at reduced size under "also" (--no-extras skips them);
+--workload runs each at full size on its own.
 """
 import argparse
 import json
@@ -247,6 +251,8 @@ def parse():
     ap.add_argument("--no-two-limb", action="store_true")
     ap.add_argument("--no-gemm-ref", action="store_true",
                     help="skip the library int8 GEMM reference point")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the reduced-size batching-form and descent runs of the default line")
     ap.add_argument("--no-ceiling", action="store_true",
                     help="skip the power-capped int8 MFMA calibration (hvk_calib_mfma_i8)")
     ap.add_argument("--no-power", action="store_true",
@@ -497,7 +503,38 @@ def main():
         "db_encode_s": db_encode_s,
         "cpu_baseline": cpu,
     }
+    if not a.no_extras:
+        line["also"] = extras(a)
     print(json.dumps(line), flush=True)
+
+
+def extras(a):
+    """The batching form and the hierarchical descent at reduced size, run after
+    the headline measurement (outside its timed region) so that one default run
+    records them too: 1M reads of the batching form's length mix (its full
+    workload is 4M), and 262,144 reads of the descent (its full workload is 1M).
+    Each keeps its own checks: the batching form's top-5 equal to the direct
+    path, the descent's batching form equal to the direct descent."""
+    import copy
+    out = {}
+    b = copy.copy(a)
+    b.no_cpu = True
+    b.batch_reads, b.batch_sample = 1 << 20, 1 << 18
+    b.descent_reads = 1 << 18
+    print("[bench] extras: batching form, 1M reads", file=sys.stderr, flush=True)
+    bl = run_batch(b, emit=False)
+    out["batch_form"] = {"workload": bl["config"]["workload"], "reads_per_s": bl["value"],
+                         "ms_per_step": bl["ms_per_step"], "steps": bl["steps"],
+                         "autotune": {k: v for k, v in bl["autotune"].items() if k != "table"},
+                         "direct_path": bl["direct_path"], "n_rescored": bl["n_rescored"]}
+    print("[bench] extras: descent, 262,144 reads", file=sys.stderr, flush=True)
+    dl = run_descent(b, emit=False)
+    out["descent"] = {"workload": dl["config"]["workload"], "reads_per_s": dl["value"],
+                      "ms_per_step": dl["ms_per_step"], "steps": dl["steps"],
+                      "encode_ms": dl["encode_ms"], "descent_ms": dl["descent_ms"],
+                      "leaf_accuracy": dl["leaf_accuracy"],
+                      "batch_form": dl["batch_form"]}
+    return out
 
 
 def encode_line(enc_ms, enc_gbs, nq, lens):
@@ -640,7 +677,7 @@ def run_precision(a):
     print(json.dumps(line), flush=True)
 
 
-def run_descent(a):
+def run_descent(a, emit=True):
     """Hierarchical descent: a 7-rank taxonomy (1 root, 40 phyla, 200 classes, 800 orders,
     3,000 families, 12,000 genera, 60,000 species leaves of 1,500 nt, each 1% from its genus
     centroid), parent prototypes the integer sums of their children's bundles, 1M reads of
@@ -780,10 +817,12 @@ def run_descent(a):
         "batch_form": batch_line,
         "cpu_baseline": cpu,
     }
-    print(json.dumps(line), flush=True)
+    if emit:
+        print(json.dumps(line), flush=True)
+    return line
 
 
-def run_batch(a):
+def run_batch(a, emit=True):
     """Batching form: 100k references (2,000 ancestors x 50), 4M reads of log-uniform length
     100-1,600 nt (clamped to the source reference) with 1% substitutions, top-5.  Reads are
     ordered by length into chunks, each chunk's encode -> MFMA search -> finalize -> scatter is
@@ -869,7 +908,9 @@ def run_batch(a):
         "n_rescored": int(out.n_rescored.item()),
         "cpu_baseline": cpu,
     }
-    print(json.dumps(line), flush=True)
+    if emit:
+        print(json.dumps(line), flush=True)
+    return line
 
 
 def run_config0(a):
