@@ -59,7 +59,7 @@ struct hvk_ctx {
     hvk::Workspace ws_search;           // partial top lists
     hvk::Workspace ws_norms;            // inverse norms
     hvk::Workspace ws_flags;            // rescan queues
-    hvk::Workspace ws_search2;          // partial lists of the exact second pass
+    hvk::Workspace ws_search2;          // partial lists of the second pass
     hvk::Workspace ws_desc;             // hierarchical descent: beams, scores, buckets
     hvk::Workspace ws_desc_tree;        // hierarchical descent: uploaded candidate lists
     hvk::Workspace ws_select;           // large-k selection: key matrix and sort buffers

@@ -2,7 +2,7 @@
 // with its workspace in a fixed device array, for callers that cannot
 // allocate device memory (code under hipGraph capture, serving loops that
 // forbid hipMalloc).  Nothing here allocates: the search buffers (inverse
-// norms, candidate lists, queues, the exact second pass's lists) are laid
+// norms, candidate lists, queues, the second pass's lists) are laid
 // out in `scratch`, a file-scope device array each GPU holds once.
 //
 // Every call on one GPU shares that GPU's array and leaves its contents

@@ -1176,14 +1176,14 @@ struct SearchBufs {
     uint64_t* qthr;       // per query: the largest threshold key any slice has reached
 };
 
-constexpr int kPass2Cap = 16384;  // queries the exact second pass takes at once
-// reference slices of the second pass (4,126 queued reads x 100k references:
-// 21.0 ms with 16 slices, 18.8 with 32, 21.0 with 64)
+constexpr int kPass2Cap = 16384;  // queries the second pass takes at once
+// reference slices of the second pass (4,126 queued reads x 100k references,
+// two-limb lists: 21.0 ms with 16 slices, 18.8 with 32, 21.0 with 64)
 constexpr int kPass2Splits = 32;
 constexpr int kPass2KP = 32;
 
 // Bytes of each search buffer for nq queries, nr references, S slices of
-// kp-entry lists; the exact second pass takes min(nq, kPass2Cap) queries.
+// kp-entry lists; the second pass takes min(nq, kPass2Cap) queries.
 struct SearchSizes {
     size_t norms, part, flags, part2;
 };
@@ -1245,7 +1245,7 @@ inline int choose_splits(hvk_ctx* ctx, int q_tiles, int r_tiles, int wg_per_cu) 
 // Automatic choice: 16 slices from 2048 query tiles up, else 8, or as many as
 // fill the device when there are few query tiles (every slice rebuilds its
 // candidate lists from an empty start, while few slices leave the last round
-// of workgroups part-empty).  want_splits > 0 (the exact second pass) or the
+// of workgroups part-empty).  want_splits > 0 (the second pass) or the
 // context's tuning value forces the slice count.
 constexpr int kGroupSlices = 8;  // reference slices per XCD group by default
 struct MfmaPlan {
@@ -1328,8 +1328,9 @@ int select_topk_limb(hvk_ctx* ctx, const int8_t* qlo, const int8_t* qhi, const i
                      int k, int32_t* idx, float* score, double* cosv, int32_t* n_res,
                      hipStream_t st);
 
-// Exact second pass of the bounded form over the queries its first
-// finalize queued, then the FP64 re-scan of what remains.
+// Second pass of the bounded form (one limb, 32-entry lists) over the
+// queries its first finalize queued, its own finalize, then the FP64 re-scan
+// of what remains.
 int exact_pass2(hvk_ctx* ctx, const hvk_bundles& q, const hvk_bundles& r, int64_t ld, int k,
                 const SearchBufs& b, int32_t* idx, float* score, double* cosv, int32_t* n_res,
                 hipStream_t st);

@@ -6,20 +6,22 @@ namespace hvk {
 HVK_SEARCH_RUNS(32, )
 HVK_SEARCH_BOUNDED(32, )
 
-// exact second pass of the bounded form over the queued queries (indirect
-// rows, device-side count): MFMA over the stored limbs, KP = kPass2KP
-template <int QL, int RL>
+// second pass of the bounded form over the queued queries (indirect rows,
+// device-side count): the same one-limb bounded MFMA (clamped rows, exact
+// bound terms bq / br of the first pass), KP = kPass2KP lists.  Its keys are
+// upper bounds like the first pass's, so finalize settles a query once its
+// k-th exact score clears the list's last key, and queues the rest for the
+// FP64 re-scan.
 void launch_pass2(hvk_ctx* ctx, const hvk_bundles& q, const hvk_bundles& r, int64_t ld, int cap,
                   const SearchBufs& b, int& n_splits, hipStream_t st) {
-    using T = MfmaTile<QL, RL>;
+    using T = MfmaTile<1, 1>;
     const MfmaPlan pl = plan_mfma(ctx, (cap + T::MT - 1) / T::MT, (r.n + T::NT - 1) / T::NT,
                                   kPass2Splits);
     n_splits = pl.map.n_splits;
-    hipLaunchKernelGGL((mfma_kernel<QL, RL, kPass2KP, false, true>), dim3(pl.blocks),
-                       dim3(kMfmaThreads), 0, st,
-                       q.lo, QL > 1 ? q.hi : nullptr, b.iq, cap, r.lo, RL > 1 ? r.hi : nullptr,
-                       b.ir, r.n, ld, pl.map, b.part2, nullptr, nullptr, b.flag_list,
-                       b.flag_count, nullptr);
+    hipLaunchKernelGGL((mfma_kernel<1, 1, kPass2KP, true, true>), dim3(pl.blocks),
+                       dim3(kMfmaThreads), 0, st, q.clamp8, nullptr, b.iq, cap, r.clamp8, nullptr,
+                       b.ir, r.n, ld, pl.map, b.part2, b.bq, b.br, b.flag_list, b.flag_count,
+                       nullptr);
 }
 
 int exact_pass2(hvk_ctx* ctx, const hvk_bundles& q, const hvk_bundles& r, int64_t ld, int k,
@@ -27,13 +29,9 @@ int exact_pass2(hvk_ctx* ctx, const hvk_bundles& q, const hvk_bundles& r, int64_
                 hipStream_t st) {
     const int nq = q.n, nr = r.n;
     const RowsLimb rows{q.lo, q.hi, r.lo, r.hi, ld};
-    const bool q2 = q.hi != nullptr, r2 = r.hi != nullptr;
     const int cap = nq < kPass2Cap ? nq : kPass2Cap;
     int S2 = 1;
-    if (q2 && r2) launch_pass2<2, 2>(ctx, q, r, ld, cap, b, S2, st);
-    else if (q2) launch_pass2<2, 1>(ctx, q, r, ld, cap, b, S2, st);
-    else if (r2) launch_pass2<1, 2>(ctx, q, r, ld, cap, b, S2, st);
-    else launch_pass2<1, 1>(ctx, q, r, ld, cap, b, S2, st);
+    launch_pass2(ctx, q, r, ld, cap, b, S2, st);
     HVK_HIP(hipGetLastError());
     hipLaunchKernelGGL((finalize_kernel<kPass2KP, RowsLimb>), dim3((nq + 3) / 4), dim3(256), 0, st,
                        rows, b.part2, S2, nq, q.sqnorm, r.sqnorm, nr, k, idx, score, cosv,
