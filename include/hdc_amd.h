@@ -253,7 +253,9 @@ typedef struct {
  *   |dot - dot_clamped| <= min(max|q|,127) resid1(r) + resid1(q) max|r|,
  * so the candidates kept are those whose upper bound ranks high.  Their exact
  * dots are recomputed from the limbs and re-scored in FP64, with the same
- * margin check and exact re-scan as hvk_search_mfma. */
+ * margin check as hvk_search_mfma; reads that fail it take a second bounded
+ * pass with 32-entry lists (d_n_rescored counts them), and what still fails
+ * takes the exact FP64 re-scan. */
 int hvk_search_mfma_bounded(hvk_ctx* ctx, const hvk_bundles* q, const hvk_bundles* r, int64_t ld,
                             int k, int32_t* d_idx, float* d_score, double* d_cos64,
                             int32_t* d_n_rescored, hvk_stream stream);
