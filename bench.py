@@ -512,14 +512,17 @@ def extras(a):
     """The batching form and the hierarchical descent at reduced size, run after
     the headline measurement (outside its timed region) so that one default run
     records them too: 1M reads of the batching form's length mix (its full
-    workload is 4M), and 262,144 reads of the descent (its full workload is 1M).
+    workload is 4M; tuned on a 524,288-read sample), and 262,144 reads of the
+    descent (its full workload is 1M).
     Each keeps its own checks: the batching form's top-5 equal to the direct
     path, the descent's batching form equal to the direct descent."""
     import copy
     out = {}
     b = copy.copy(a)
     b.no_cpu = True
-    b.batch_reads, b.batch_sample = 1 << 20, 1 << 18
+    # a half-size sample lets the tuner reach 524,288-read chunks (two per step), the
+    # size at which a chunk's search runs the same 16 slices as the direct path
+    b.batch_reads, b.batch_sample = 1 << 20, 1 << 19
     b.descent_reads = 1 << 18
     print("[bench] extras: batching form, 1M reads", file=sys.stderr, flush=True)
     bl = run_batch(b, emit=False)
